@@ -1,0 +1,318 @@
+// Dictionary projection and experimental-pattern preparation for gfx950.
+//
+// ebsd_project fuses, per (orientation, detector pixel):
+//   quaternion rotation of the detector ray       (quaternions.py:186-213)
+//   side-by-side Rosca-Lambert mapping            (sphere.py:145-185)
+//   bilinear grid_sample, border, corners=False   (master_pattern.py:197-206)
+// and, per orientation, the mean subtraction of indexing.py:139-143 plus the
+// optional unit norm / low-precision cast used by the precision forms.
+//
+// One 256-thread workgroup owns one pattern. Its pixels live in registers
+// (ITEMS per thread) between the sampling pass and the normalisation pass, so
+// every output byte is written exactly once and coalesced. The master pattern
+// (2 x S^2 floats, 2 MB at S = 501) stays resident in each XCD's L2.
+//
+// Floating-point contraction is disabled for this file (see the build
+// recipe) so that each arithmetic step rounds like the step-by-step torch
+// expressions it mirrors.
+
+#include "common.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_fp8.h>
+
+namespace {
+
+constexpr float kFp8Max = 448.0f;  // largest finite OCP e4m3fn value
+
+// Rotate p by unit quaternion (w, x, y, z) with the exact operation order of
+// qu_apply (quaternions.py:203-211): q * (1, p) * conj(q), vector part.
+__device__ __forceinline__ void rotate_ray(float aw, float ax, float ay, float az,
+                                           float bx, float by, float bz,
+                                           float& ox, float& oy, float& oz) {
+  const float iw = ((aw - ax * bx) - ay * by) - az * bz;
+  const float ix = ((aw * bx + ax) + ay * bz) - az * by;
+  const float iy = ((aw * by - ax * bz) + ay) + az * bx;
+  const float iz = ((aw * bz + ax * by) - ay * bx) + az;
+  ox = (((-iw) * ax + ix * aw) - iy * az) + iz * ay;
+  oy = (((-iw) * ay + ix * az) + iy * aw) - iz * ax;
+  oz = (((-iw) * az - ix * ay) + iy * ax) + iz * aw;
+}
+
+// Side-by-side Rosca-Lambert coordinates in grid_sample's [-1, 1] frame
+// (sphere.py:161-185): gx spans the 2S-wide image (north half left), gy the
+// S rows.
+__device__ __forceinline__ void lambert_sbs(float x, float y, float z, float& gx, float& gy) {
+  const float factor = sqrtf(fmaxf(2.0f * (1.0f - fabsf(z)), 0.0f));
+  const bool cond = fabsf(y) <= fabsf(x);
+  const float big = cond ? x : y;
+  const float sml = cond ? y : x;
+  const float sgn = (big < 0.0f) ? -1.0f : 1.0f;
+  const float sf = sgn * factor;
+  const float simpler = sf * 0.70710678118654752440f;              // 2 / sqrt(8)
+  const float arct = (sf * atan2f(sml * sgn, fabsf(big))) * 0.90031631615710606956f;  // 2 sqrt(2) / pi
+  const float u = cond ? simpler : arct;
+  gy = cond ? arct : simpler;
+  gx = ((u / 2.0f) - 0.5f) + ((z < 0.0f) ? 1.0f : 0.0f);
+}
+
+// Bilinear sample of an (H, W) image at normalised (gx, gy) with border
+// padding and align_corners=False: source index ((g + 1) * size - 1) / 2,
+// clipped to [0, size - 1]; the four taps are blended as
+// nw*v_nw + ne*v_ne + sw*v_sw + se*v_se, in that order.
+__device__ __forceinline__ float sample_border(const float* __restrict__ img, int H, int W,
+                                               float gx, float gy) {
+  float ix = (gx + 1.0f) * (0.5f * (float)W) - 0.5f;
+  float iy = (gy + 1.0f) * (0.5f * (float)H) - 0.5f;
+  ix = fminf(fmaxf(ix, 0.0f), (float)(W - 1));
+  iy = fminf(fmaxf(iy, 0.0f), (float)(H - 1));
+  const float x0f = floorf(ix);
+  const float y0f = floorf(iy);
+  const int x0 = (int)x0f;
+  const int y0 = (int)y0f;
+  // A tap past the last column/row carries weight exactly 0; clamp its
+  // address so it stays in bounds.
+  const int x1 = min(x0 + 1, W - 1);
+  const int y1 = min(y0 + 1, H - 1);
+  const float w = ix - x0f;
+  const float e = 1.0f - w;
+  const float n = iy - y0f;
+  const float s = 1.0f - n;
+  const float v_nw = __ldg(img + (size_t)y0 * W + x0);
+  const float v_ne = __ldg(img + (size_t)y0 * W + x1);
+  const float v_sw = __ldg(img + (size_t)y1 * W + x0);
+  const float v_se = __ldg(img + (size_t)y1 * W + x1);
+  return ((v_nw * (s * e) + v_ne * (s * w)) + v_sw * (n * e)) + v_se * (n * w);
+}
+
+template <typename T>
+__device__ __forceinline__ T to_out(float v);
+template <>
+__device__ __forceinline__ float to_out<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_out<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+template <>
+__device__ __forceinline__ __half to_out<__half>(float v) { return __float2half(v); }
+template <>
+__device__ __forceinline__ uint8_t to_out<uint8_t>(float v) {
+  return (uint8_t)__hip_cvt_float_to_fp8(v, __HIP_SATFINITE, __HIP_E4M3);
+}
+
+template <typename T> struct is_fp8 { static constexpr bool value = false; };
+template <> struct is_fp8<uint8_t> { static constexpr bool value = true; };
+
+// Normalise the ITEMS register values of one pattern (n_valid of them are
+// real pixels) and store row `row` of `out`, zero-padding to ld.
+template <int ITEMS, typename OutT>
+__device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, int flags,
+                                           OutT* __restrict__ out, int ld, float* __restrict__ out_scale,
+                                           size_t row, float* red) {
+  const int tid = threadIdx.x;
+  if (flags & EBSD_FLAG_ZERO_MEAN) {
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) s += vals[i];  // padding slots hold 0
+    const float mean = block_sum_256(s, red) / (float)n_valid;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int d = tid + i * 256;
+      vals[i] = (d < n_valid) ? (vals[i] - mean) : 0.0f;
+    }
+  }
+  if (flags & EBSD_FLAG_UNIT_NORM) {
+    float ss = 0.0f;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) ss += vals[i] * vals[i];
+    const float tot = block_sum_256(ss, red);
+    const float inv = tot > 0.0f ? 1.0f / sqrtf(tot) : 0.0f;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) vals[i] *= inv;
+  }
+  float mul = 1.0f;
+  if constexpr (is_fp8<OutT>::value) {
+    float am = 0.0f;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) am = fmaxf(am, fabsf(vals[i]));
+    am = block_max_256(am, red);
+    const float scale = am > 0.0f ? am / kFp8Max : 1.0f;
+    mul = 1.0f / scale;
+    if (tid == 0 && out_scale) out_scale[row] = scale;
+  } else {
+    if (tid == 0 && out_scale) out_scale[row] = 1.0f;
+  }
+  OutT* orow = out + row * (size_t)ld;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const int d = tid + i * 256;
+    if (d < ld) orow[d] = to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f);
+  }
+  // Columns beyond ITEMS*256 (only when ld exceeds it) are zero padding.
+  for (int d = ITEMS * 256 + tid; d < ld; d += 256) orow[d] = to_out<OutT>(0.0f);
+}
+
+template <int ITEMS, typename OutT>
+__global__ __launch_bounds__(256) void project_kernel(
+    const float* __restrict__ mp, int S, const float* __restrict__ rays, int n_pix,
+    const float* __restrict__ quats, OutT* __restrict__ out, int ld,
+    float* __restrict__ out_scale, int flags) {
+  __shared__ float red[4];
+  const size_t n = blockIdx.x;
+  const float4 q = *reinterpret_cast<const float4*>(quats + 4 * n);
+  const int W = 2 * S;
+  const bool renorm = (flags & EBSD_FLAG_RENORM_DIRS) != 0;
+  float vals[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const int d = threadIdx.x + i * 256;
+    float v = 0.0f;
+    if (d < n_pix) {
+      const float bx = __ldg(rays + 3 * d + 0);
+      const float by = __ldg(rays + 3 * d + 1);
+      const float bz = __ldg(rays + 3 * d + 2);
+      float x, y, z;
+      rotate_ray(q.x, q.y, q.z, q.w, bx, by, bz, x, y, z);
+      if (renorm) {
+        const float nr = sqrtf((x * x + y * y) + z * z);
+        x = x / nr;
+        y = y / nr;
+        z = z / nr;
+      }
+      float gx, gy;
+      lambert_sbs(x, y, z, gx, gy);
+      v = sample_border(mp, S, W, gx, gy);
+    }
+    vals[i] = v;
+  }
+  finish_row<ITEMS, OutT>(vals, n_pix, flags, out, ld, out_scale, n, red);
+}
+
+template <int ITEMS, typename OutT>
+__global__ __launch_bounds__(256) void prep_kernel(
+    const float* __restrict__ pats, int ld_in, const int* __restrict__ pix_idx, int n_pix,
+    OutT* __restrict__ out, int ld, float* __restrict__ out_scale, int flags) {
+  __shared__ float red[4];
+  const size_t n = blockIdx.x;
+  const float* row = pats + n * (size_t)ld_in;
+  float vals[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const int d = threadIdx.x + i * 256;
+    float v = 0.0f;
+    if (d < n_pix) v = row[pix_idx ? pix_idx[d] : d];
+    vals[i] = v;
+  }
+  finish_row<ITEMS, OutT>(vals, n_pix, flags, out, ld, out_scale, n, red);
+}
+
+template <typename OutT>
+int launch_project(const float* mp, int S, const float* rays, int n_pix, const float* quats,
+                   int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st) {
+  const dim3 grid(n_ori), block(256);
+  OutT* o = static_cast<OutT*>(out);
+  const int items = (n_pix + 255) / 256;
+#define EBSD_PROJ_CASE(I)                                                                        \
+  if (items <= I) {                                                                              \
+    hipLaunchKernelGGL((project_kernel<I, OutT>), grid, block, 0, st, mp, S, rays, n_pix, quats, \
+                       o, ld, out_scale, flags);                                                 \
+    EBSD_LAUNCH_CHECK();                                                                         \
+    return EBSD_OK;                                                                              \
+  }
+  EBSD_PROJ_CASE(4)
+  EBSD_PROJ_CASE(8)
+  EBSD_PROJ_CASE(16)
+  EBSD_PROJ_CASE(32)
+  EBSD_PROJ_CASE(64)
+#undef EBSD_PROJ_CASE
+  return EBSD_ERR_SHAPE;
+}
+
+template <typename OutT>
+int launch_prep(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix, void* out,
+                int ld, float* out_scale, int flags, hipStream_t st) {
+  const dim3 grid(n), block(256);
+  OutT* o = static_cast<OutT*>(out);
+  const int items = (n_pix + 255) / 256;
+#define EBSD_PREP_CASE(I)                                                                      \
+  if (items <= I) {                                                                            \
+    hipLaunchKernelGGL((prep_kernel<I, OutT>), grid, block, 0, st, pats, ld_in, pix_idx, n_pix, \
+                       o, ld, out_scale, flags);                                               \
+    EBSD_LAUNCH_CHECK();                                                                       \
+    return EBSD_OK;                                                                            \
+  }
+  EBSD_PREP_CASE(4)
+  EBSD_PREP_CASE(8)
+  EBSD_PREP_CASE(16)
+  EBSD_PREP_CASE(32)
+  EBSD_PREP_CASE(64)
+#undef EBSD_PREP_CASE
+  return EBSD_ERR_SHAPE;
+}
+
+}  // namespace
+
+extern "C" int ebsd_pad_multiple(int dtype) {
+  switch (dtype) {
+    case EBSD_DTYPE_F32: return 32;
+    case EBSD_DTYPE_BF16:
+    case EBSD_DTYPE_F16: return 64;
+    case EBSD_DTYPE_FP8_E4M3: return 128;
+    default: return EBSD_ERR_DTYPE;
+  }
+}
+
+static int check_ld(int dtype, int n_pix, int ld) {
+  const int pm = ebsd_pad_multiple(dtype);
+  if (pm < 0) return EBSD_ERR_DTYPE;
+  if (n_pix <= 0 || n_pix > 64 * 256) return EBSD_ERR_SHAPE;
+  if (ld < n_pix || ld % pm != 0) return EBSD_ERR_SHAPE;
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const float* rays,
+                            int n_pix, const float* quats, int n_ori, void* out, int out_dtype,
+                            int ld_out, float* out_scale, int flags, void* stream) {
+  if (!mp || !rays || !quats || !out) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols != 2 * mp_rows || n_ori < 0) return EBSD_ERR_SHAPE;
+  const int chk = check_ld(out_dtype, n_pix, ld_out);
+  if (chk) return chk;
+  if (out_dtype == EBSD_DTYPE_FP8_E4M3 && !out_scale) return EBSD_ERR_NULL;
+  if ((reinterpret_cast<uintptr_t>(quats) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (n_ori == 0) return EBSD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (out_dtype) {
+    case EBSD_DTYPE_F32:
+      return launch_project<float>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+    case EBSD_DTYPE_BF16:
+      return launch_project<__hip_bfloat16>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+    case EBSD_DTYPE_F16:
+      return launch_project<__half>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+    case EBSD_DTYPE_FP8_E4M3:
+      return launch_project<uint8_t>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+  }
+  return EBSD_ERR_DTYPE;
+}
+
+extern "C" int ebsd_prep_patterns(const float* pats, int n, int ld_in, const int* pix_idx,
+                                  int n_pix, void* out, int out_dtype, int ld_out,
+                                  float* out_scale, int flags, void* stream) {
+  if (!pats || !out) return EBSD_ERR_NULL;
+  if (n < 0 || ld_in < 1) return EBSD_ERR_SHAPE;
+  if (!pix_idx && n_pix > ld_in) return EBSD_ERR_SHAPE;
+  const int chk = check_ld(out_dtype, n_pix, ld_out);
+  if (chk) return chk;
+  if (out_dtype == EBSD_DTYPE_FP8_E4M3 && !out_scale) return EBSD_ERR_NULL;
+  if (n == 0) return EBSD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (out_dtype) {
+    case EBSD_DTYPE_F32:
+      return launch_prep<float>(pats, n, ld_in, pix_idx, n_pix, out, ld_out, out_scale, flags, st);
+    case EBSD_DTYPE_BF16:
+      return launch_prep<__hip_bfloat16>(pats, n, ld_in, pix_idx, n_pix, out, ld_out, out_scale, flags, st);
+    case EBSD_DTYPE_F16:
+      return launch_prep<__half>(pats, n, ld_in, pix_idx, n_pix, out, ld_out, out_scale, flags, st);
+    case EBSD_DTYPE_FP8_E4M3:
+      return launch_prep<uint8_t>(pats, n, ld_in, pix_idx, n_pix, out, ld_out, out_scale, flags, st);
+  }
+  return EBSD_ERR_DTYPE;
+}

@@ -1,0 +1,299 @@
+// Similarity GEMM with a fused streaming top-k for gfx950.
+//
+// Scores S[q, d] = <Q[q, :], Dict[d, :]> (utils/knn.py:96-97, metric
+// "angular") are produced tile by tile on the matrix cores and reduced at once
+// to a running top-k per query held in VGPRs; S never reaches memory.
+//
+// Work decomposition
+//   grid.x  : query tiles of 128 rows
+//   grid.y  : dictionary splits; split s walks dictionary tiles
+//             [s * tiles_per_split, (s + 1) * tiles_per_split) of 128 rows
+//   block   : 256 threads = 4 waves; wave w owns queries [32 w, 32 w + 32)
+//             of the block and all 128 dictionary rows of the current tile,
+//             i.e. a 128 x 32 score block = 4 MFMA 32x32 tiles (64 acc VGPRs)
+// The K dimension streams through double-buffered LDS in 32-float stages;
+// the next stage is fetched into registers while the current one feeds the
+// MFMAs. After each dictionary tile, every lane folds its 64 fresh scores
+// (1 query column x 64 dictionary rows) into one register top-k list, so a
+// query is covered by two lists (the two lane halves of its wave). At the
+// end they are merged through LDS into one partial list per (query, split);
+// a second kernel merges the splits (and, optionally, the caller's running
+// state).
+
+#include "common.h"
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int TILE = 128;        // dictionary rows per tile, queries per block
+constexpr int BK32 = 32;         // f32 elements of K per LDS stage
+constexpr int LDS_STRIDE = 36;   // padded row stride (floats): conflict-free b128 reads
+constexpr int LDS_TILE = TILE * LDS_STRIDE;
+
+// Per-lane position of accumulator element r of a 32x32 MFMA tile:
+// column = lane % 32, row = 8 * (r / 4) + 4 * (lane / 32) + r % 4.
+__device__ __forceinline__ int mfma32_row(int r, int lane) {
+  return ((r >> 2) << 3) + ((lane >> 5) << 2) + (r & 3);
+}
+
+template <int K>
+__global__ __launch_bounds__(256, 2) void sim_topk_f32_kernel(
+    const float* __restrict__ Q, int M, const float* __restrict__ Dm, int N, int ld,
+    int tiles_per_split, float* __restrict__ pval, int* __restrict__ pidx, int index_base) {
+  __shared__ __attribute__((aligned(16))) float smem[4 * LDS_TILE];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int q0 = blockIdx.x * TILE;
+  const int split = blockIdx.y;
+  const int n_tiles = (N + TILE - 1) / TILE;
+  const int t_begin = split * tiles_per_split;
+  const int t_end = min(t_begin + tiles_per_split, n_tiles);
+
+  // Global->LDS mapping: thread loads rows lr + 32 i (i < 4), floats lc..lc+3
+  // of both the dictionary tile and the query tile.
+  const int lr = tid >> 3;
+  const int lc = (tid & 7) * 4;
+  const float* qrow[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) qrow[i] = Q + (size_t)min(q0 + lr + 32 * i, M - 1) * ld + lc;
+
+  float tv[K];
+  int ti[K];
+  topk_init<K>(tv, ti);
+
+  const int nk = ld / BK32;
+  // LDS read offsets (floats) of this lane inside a stage. Lane half h = lane/32
+  // supplies K elements [16 h, 16 h + 16) of the stage, four per b128 read;
+  // A and B use the same K order, so each MFMA sums matching products.
+  const int a_off = (lane & 31) * LDS_STRIDE + (lane >> 5) * 16;
+  const int b_off = (wave * 32 + (lane & 31)) * LDS_STRIDE + (lane >> 5) * 16;
+  const int st_off = lr * LDS_STRIDE + lc;
+
+  for (int t = t_begin; t < t_end; ++t) {
+    const int d0 = t * TILE;
+    const float* drow[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) drow[i] = Dm + (size_t)min(d0 + lr + 32 * i, N - 1) * ld + lc;
+
+    floatx16 acc[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][r] = 0.0f;
+
+    floatx4 ra[4], rb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ra[i] = *reinterpret_cast<const floatx4*>(drow[i]);
+      rb[i] = *reinterpret_cast<const floatx4*>(qrow[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      *reinterpret_cast<floatx4*>(smem + st_off + 32 * i * LDS_STRIDE) = ra[i];
+      *reinterpret_cast<floatx4*>(smem + LDS_TILE + st_off + 32 * i * LDS_STRIDE) = rb[i];
+    }
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = (kt + 1) < nk;
+      if (more) {
+        const int ko = (kt + 1) * BK32;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          ra[i] = *reinterpret_cast<const floatx4*>(drow[i] + ko);
+          rb[i] = *reinterpret_cast<const floatx4*>(qrow[i] + ko);
+        }
+      }
+      const float* As = smem + cur * 2 * LDS_TILE;
+      const float* Bs = As + LDS_TILE;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const floatx4 b = *reinterpret_cast<const floatx4*>(Bs + b_off + s * 4);
+        floatx4 a[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+          a[bi] = *reinterpret_cast<const floatx4*>(As + a_off + bi * 32 * LDS_STRIDE + s * 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi)
+            acc[bi] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[bi][e], b[e], acc[bi], 0, 0, 0);
+      }
+      if (more) {
+        float* Ad = smem + (cur ^ 1) * 2 * LDS_TILE;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          *reinterpret_cast<floatx4*>(Ad + st_off + 32 * i * LDS_STRIDE) = ra[i];
+          *reinterpret_cast<floatx4*>(Ad + LDS_TILE + st_off + 32 * i * LDS_STRIDE) = rb[i];
+        }
+      }
+      // Orders this stage's reads before the next overwrite of its buffer,
+      // and (last stage) before the next tile's prologue writes stage 0.
+      __syncthreads();
+    }
+
+    // Fold the tile into the running list: this lane holds query column
+    // wave*32 + lane%32 and 64 of the tile's 128 dictionary rows.
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int d = d0 + bi * 32 + mfma32_row(r, lane);
+        if (d < N) topk_offer<K>(tv, ti, acc[bi][r], index_base + d);
+      }
+    }
+  }
+
+  // Merge the 2 lists per query (the two lane halves of its wave) in LDS.
+  float* cv = smem;  // [128 queries][2 lists][K]
+  int* ci = reinterpret_cast<int*>(smem + TILE * 2 * K);
+  {
+    const int ql = wave * 32 + (lane & 31);
+    const int list = lane >> 5;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      cv[(ql * 2 + list) * K + j] = tv[j];
+      ci[(ql * 2 + list) * K + j] = ti[j];
+    }
+  }
+  __syncthreads();
+  if (tid < TILE) {
+    const int q = q0 + tid;
+    const float* v = cv + tid * 2 * K;
+    const int* ix = ci + tid * 2 * K;
+    int h0 = 0, h1 = 0;
+    for (int j = 0; j < K; ++j) {
+      // A list can only run dry on the K-th pick, so both heads stay < K
+      // whenever they are read.
+      const float v0 = v[h0], v1 = v[K + h1];
+      const int i0 = ix[h0], i1 = ix[K + h1];
+      const bool take1 = topk_better(v1, i1, v0, i0);
+      if (q < M) {
+        const size_t o = (size_t)(split * K + j) * M + q;
+        pval[o] = take1 ? v1 : v0;
+        pidx[o] = take1 ? i1 : i0;
+      }
+      h0 += take1 ? 0 : 1;
+      h1 += take1 ? 1 : 0;
+    }
+  }
+}
+
+// Merge n_splits sorted partial lists (layout [split][K][M]) and, if `merge`,
+// the caller's running state (M, topk), into the sorted state (M, topk).
+template <int K>
+__global__ __launch_bounds__(256) void topk_merge_kernel(
+    const float* __restrict__ pval, const int* __restrict__ pidx, int n_splits, int M, int topk,
+    float* __restrict__ out_val, int* __restrict__ out_idx, int merge) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= M) return;
+  float tv[K];
+  int ti[K];
+  topk_init<K>(tv, ti);
+  if (merge) {
+    for (int j = 0; j < topk; ++j) {
+      const float v = out_val[(size_t)q * topk + j];
+      const int id = out_idx[(size_t)q * topk + j];
+      topk_offer<K>(tv, ti, v, id);
+    }
+  }
+  for (int s = 0; s < n_splits; ++s) {
+    for (int j = 0; j < K; ++j) {
+      const size_t o = (size_t)(s * K + j) * M + q;
+      const float v = pval[o];
+      const int id = pidx[o];
+      if (!topk_better(v, id, tv[K - 1], ti[K - 1])) break;  // lists are sorted
+      topk_insert<K>(tv, ti, v, id);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (j < topk) {
+      out_val[(size_t)q * topk + j] = tv[j];
+      out_idx[(size_t)q * topk + j] = ti[j] == INT_MAX ? -1 : ti[j];
+    }
+  }
+}
+
+struct SimPlan {
+  int k_list;           // register list length (1 or 16)
+  int n_qtiles;
+  int n_splits;
+  int tiles_per_split;
+};
+
+SimPlan plan_sim(int m, int n, int topk) {
+  SimPlan p;
+  p.k_list = topk <= 1 ? 1 : EBSD_MAX_TOPK;
+  p.n_qtiles = (m + TILE - 1) / TILE;
+  const int n_tiles = (n + TILE - 1) / TILE;
+  // Aim for >= 2048 workgroups (256 CUs x 2 resident x 4 rounds) so the
+  // dictionary walk is balanced, but never split below one tile.
+  int splits = (2048 + p.n_qtiles - 1) / p.n_qtiles;
+  if (splits > n_tiles) splits = n_tiles;
+  if (splits < 1) splits = 1;
+  p.tiles_per_split = (n_tiles + splits - 1) / splits;
+  p.n_splits = (n_tiles + p.tiles_per_split - 1) / p.tiles_per_split;
+  return p;
+}
+
+size_t workspace_bytes(const SimPlan& p, int m) {
+  return (size_t)p.n_splits * p.k_list * (size_t)m * (sizeof(float) + sizeof(int));
+}
+
+}  // namespace
+
+extern "C" int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t* bytes) {
+  if (!bytes) return EBSD_ERR_NULL;
+  if (m < 0 || n < 1) return EBSD_ERR_SHAPE;
+  if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
+  if (dtype != EBSD_DTYPE_F32) return EBSD_ERR_DTYPE;
+  const SimPlan p = plan_sim(m, n, topk);
+  *bytes = workspace_bytes(p, m);
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_sim_topk(const void* q, const float* q_scale, int m, const void* dict,
+                             const float* d_scale, int n, int ld, int dtype, int topk,
+                             float* top_val, int* top_idx, int index_base, int merge,
+                             void* workspace, size_t ws_bytes, void* stream) {
+  (void)q_scale;
+  (void)d_scale;
+  if (!q || !dict || !top_val || !top_idx) return EBSD_ERR_NULL;
+  if (m < 0 || n < 1 || ld < 1) return EBSD_ERR_SHAPE;
+  if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
+  if (dtype != EBSD_DTYPE_F32) return EBSD_ERR_DTYPE;
+  if (ld % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
+  if ((int64_t)index_base + n > (int64_t)INT_MAX) return EBSD_ERR_SHAPE;
+  if (((reinterpret_cast<uintptr_t>(q) | reinterpret_cast<uintptr_t>(dict)) & 15) != 0)
+    return EBSD_ERR_ALIGN;
+  if (m == 0) return EBSD_OK;
+  const SimPlan p = plan_sim(m, n, topk);
+  if (!workspace || ws_bytes < workspace_bytes(p, m)) return EBSD_ERR_WORKSPACE;
+  float* pval = static_cast<float*>(workspace);
+  int* pidx = reinterpret_cast<int*>(pval + (size_t)p.n_splits * p.k_list * m);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid(p.n_qtiles, p.n_splits), block(256);
+  const float* Qf = static_cast<const float*>(q);
+  const float* Df = static_cast<const float*>(dict);
+  const dim3 mgrid((m + 255) / 256);
+  if (p.k_list == 1) {
+    hipLaunchKernelGGL(sim_topk_f32_kernel<1>, grid, block, 0, st, Qf, m, Df, n, ld,
+                       p.tiles_per_split, pval, pidx, index_base);
+    EBSD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(topk_merge_kernel<1>, mgrid, dim3(256), 0, st, pval, pidx, p.n_splits, m,
+                       topk, top_val, top_idx, merge);
+  } else {
+    hipLaunchKernelGGL(sim_topk_f32_kernel<EBSD_MAX_TOPK>, grid, block, 0, st, Qf, m, Df, n, ld,
+                       p.tiles_per_split, pval, pidx, index_base);
+    EBSD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(topk_merge_kernel<EBSD_MAX_TOPK>, mgrid, dim3(256), 0, st, pval, pidx,
+                       p.n_splits, m, topk, top_val, top_idx, merge);
+  }
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
