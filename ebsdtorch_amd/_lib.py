@@ -56,11 +56,16 @@ _SIGNATURES = {
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p,
          _c_int, _c_int, _c_void_p, _c_int, _c_void_p],
     ),
+    "ebsd_sample_dirs": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, ctypes.c_longlong, _c_void_p, _c_int, _c_void_p]),
     "ebsd_prep_patterns": (
         _c_int,
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int,
          _c_void_p, _c_int, _c_void_p],
     ),
+    "ebsd_rescore": (
+        _c_int,
+        [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p]),
     "ebsd_sim_topk_workspace": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_c_size_t)]),
     "ebsd_sim_topk": (
         _c_int,

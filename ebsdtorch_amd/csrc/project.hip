@@ -205,6 +205,24 @@ __global__ __launch_bounds__(256) void prep_kernel(
   finish_row<ITEMS, OutT>(vals, n_pix, flags, out, ld, out_scale, n, red);
 }
 
+// Sample the master pattern at arbitrary sphere directions (one per thread).
+__global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restrict__ mp, int S,
+                                                          const float* __restrict__ dirs, int64_t n,
+                                                          float* __restrict__ out, int renorm) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+  if (renorm) {
+    const float nr = sqrtf((x * x + y * y) + z * z);
+    x = x / nr;
+    y = y / nr;
+    z = z / nr;
+  }
+  float gx, gy;
+  lambert_sbs(x, y, z, gx, gy);
+  out[i] = sample_border(mp, S, 2 * S, gx, gy);
+}
+
 template <typename OutT>
 int launch_project(const float* mp, int S, const float* rays, int n_pix, const float* quats,
                    int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st) {
@@ -291,6 +309,19 @@ extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const flo
       return launch_project<uint8_t>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
   }
   return EBSD_ERR_DTYPE;
+}
+
+extern "C" int ebsd_sample_dirs(const float* mp, int mp_rows, int mp_cols, const float* dirs,
+                                long long n, float* out, int renorm, void* stream) {
+  if (!mp || !dirs || !out) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols != 2 * mp_rows || n < 0) return EBSD_ERR_SHAPE;
+  if (n == 0) return EBSD_OK;
+  const int64_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffffLL) return EBSD_ERR_SHAPE;
+  hipLaunchKernelGGL(sample_dirs_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), mp, mp_rows, dirs, (int64_t)n, out, renorm);
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
 }
 
 extern "C" int ebsd_prep_patterns(const float* pats, int n, int ld_in, const int* pix_idx,

@@ -219,6 +219,42 @@ __global__ __launch_bounds__(256) void topk_merge_kernel(
   }
 }
 
+// FP32 re-scoring of candidate lists: one wave per query computes the exact
+// FP32 dot product with each candidate row, then keeps the best `topk`.
+template <int C>
+__global__ __launch_bounds__(256) void rescore_kernel(
+    const float* __restrict__ Q, int M, const float* __restrict__ Dm, int N, int ld,
+    const int* __restrict__ cand, int n_cand, int topk, float* __restrict__ out_val,
+    int* __restrict__ out_idx) {
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= M) return;
+  const floatx4* qr = reinterpret_cast<const floatx4*>(Q + (size_t)q * ld);
+  const int n4 = ld / 4;
+  float tv[C];
+  int ti[C];
+  topk_init<C>(tv, ti);
+  for (int j = 0; j < n_cand; ++j) {
+    const int id = cand[(size_t)q * n_cand + j];  // wave-uniform
+    if (id < 0 || id >= N) continue;  // empty slot or not a dictionary row
+    const floatx4* dr = reinterpret_cast<const floatx4*>(Dm + (size_t)id * ld);
+    float acc = 0.0f;
+    for (int c = lane; c < n4; c += 64) {
+      const floatx4 a = qr[c];
+      const floatx4 b = dr[c];
+      acc += a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+    }
+    acc = wave_sum(acc);
+    topk_offer<C>(tv, ti, acc, id);
+  }
+  if (lane == 0) {
+    for (int j = 0; j < topk; ++j) {
+      out_val[(size_t)q * topk + j] = tv[j];
+      out_idx[(size_t)q * topk + j] = ti[j] == INT_MAX ? -1 : ti[j];
+    }
+  }
+}
+
 struct SimPlan {
   int k_list;           // register list length (1 or 16)
   int n_qtiles;
@@ -246,6 +282,20 @@ size_t workspace_bytes(const SimPlan& p, int m) {
 }
 
 }  // namespace
+
+extern "C" int ebsd_rescore(const float* q, int m, const float* dict, int n, int ld, const int* cand,
+                            int n_cand, int topk, float* out_val, int* out_idx, void* stream) {
+  if (!q || !dict || !cand || !out_val || !out_idx) return EBSD_ERR_NULL;
+  if (m < 0 || n < 1 || ld < 1 || ld % 32 != 0) return EBSD_ERR_SHAPE;
+  if (n_cand < 1 || n_cand > EBSD_MAX_TOPK || topk < 1 || topk > n_cand) return EBSD_ERR_TOPK;
+  if (((reinterpret_cast<uintptr_t>(q) | reinterpret_cast<uintptr_t>(dict)) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (m == 0) return EBSD_OK;
+  hipLaunchKernelGGL(rescore_kernel<EBSD_MAX_TOPK>, dim3((m + 3) / 4), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const float*>(q), m,
+                     static_cast<const float*>(dict), n, ld, cand, n_cand, topk, out_val, out_idx);
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
 
 extern "C" int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t* bytes) {
   if (!bytes) return EBSD_ERR_NULL;

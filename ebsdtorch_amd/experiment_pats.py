@@ -1,0 +1,119 @@
+"""Experimental pattern container (ebsd/experiment_pats.py:19-451): stores
+the scan, hands out (binned) pattern chunks and keeps indexing results."""
+
+from __future__ import annotations
+
+from typing import List, Optional
+
+import torch
+
+
+class ExperimentPatterns(torch.nn.Module):
+    def __init__(self, patterns: torch.Tensor):
+        super().__init__()
+        if patterns.dim() < 2:
+            raise ValueError("'patterns' requires >= 2 dimensions (B, H, W) or (H, W)")
+        if patterns.dim() < 3:
+            patterns = patterns.unsqueeze(0)
+        if patterns.dim() > 4:
+            raise ValueError("3D Volume not yet supported... 'patterns' requires dimensions: "
+                             "(SCAN_H, SCAN_W, H, W), (N_PATS, H, W), or (H, W)")
+        self.pattern_shape = tuple(patterns.shape[-2:])
+        self.spatial_shape = tuple(patterns.shape[:-2]) if patterns.dim() > 2 else (1, 1)
+        self.patterns = patterns.reshape(-1, *self.pattern_shape)
+        self.n_patterns = self.patterns.shape[0]
+        self.n_pixels = self.pattern_shape[0] * self.pattern_shape[1]
+        self.phases = None
+        self.orientations = None
+        self.best_metrics = None
+        self.spatial_coords = None
+        self.indexing_indices = None
+
+    def set_spatial_coords(self, spatial_coords: torch.Tensor, indices: Optional[torch.Tensor] = None):
+        if indices is None:
+            if spatial_coords.shape[0] != self.n_patterns:
+                raise ValueError("Spatial coordinates must have the same number of patterns as the "
+                                 f"ExperimentPatterns object. Got {spatial_coords.shape[0]} and {self.n_patterns}.")
+            self.spatial_coords = spatial_coords
+        else:
+            if spatial_coords.shape[0] != indices.shape[0]:
+                raise ValueError("Spatial coordinates must have the same number of patterns as the indices.")
+            self.spatial_coords[indices] = spatial_coords
+
+    def get_spatial_coords(self, indices: Optional[torch.Tensor] = None) -> torch.Tensor:
+        if self.spatial_coords is None:
+            raise ValueError("Spatial coordinates must be set before retrieving them.")
+        return self.spatial_coords if indices is None else self.spatial_coords[indices]
+
+    def set_orientations(self, orientations: torch.Tensor, indices: Optional[torch.Tensor] = None):
+        if orientations.dim() != 2:
+            raise ValueError(f"Orientations must be quaternions (N_ORI, 4). Got {tuple(orientations.shape)}.")
+        if orientations.shape[1] != 4:
+            raise ValueError(f"Orientations must be quaternions (w, x, y, z). Got {orientations.shape[1]}.")
+        if indices is None:
+            if orientations.shape[0] != self.n_patterns:
+                raise ValueError("Orientations must have the same number of patterns as the ExperimentPatterns "
+                                 f"object. Got {orientations.shape[0]} orientations and {self.n_patterns} patterns.")
+            self.orientations = orientations
+        else:
+            if indices.dim() != 1:
+                raise ValueError(f"Indices must be (N_ORI,). Got {tuple(indices.shape)}.")
+            if orientations.shape[0] != indices.shape[0]:
+                raise ValueError("Orientations must have the same number of patterns as the indices.")
+            self.orientations[indices] = orientations
+
+    def get_orientations(self, indices: Optional[torch.Tensor] = None) -> torch.Tensor:
+        return self.orientations if indices is None else self.orientations[indices]
+
+    def get_patterns(self, indices: torch.Tensor, binning: int) -> torch.Tensor:
+        """Patterns at `indices`, average-pooled by `binning`
+        (experiment_pats.py:318-346)."""
+        pats = self.patterns[indices]
+        if binning > 1:
+            pats = torch.nn.functional.avg_pool2d(pats, binning)
+        return pats
+
+    def get_indices_per_phase(self) -> List[torch.Tensor]:
+        if self.phases is None:
+            raise ValueError("Phases must be indexed before retrieving indices per phase.")
+        return [torch.nonzero(self.phases == i, as_tuple=False).squeeze()
+                for i in range(int(self.phases.max().item()) + 1)]
+
+    def set_raw_indexing_results(self, quaternions: torch.Tensor, metrics: torch.Tensor, phase_id: int,
+                                 indices: Optional[torch.Tensor] = None):
+        """Store one phase's (N, k, 4) quaternions and (N, k) metrics
+        (experiment_pats.py:367-385); `indices` additionally keeps the (N, k)
+        dictionary indices."""
+        if not hasattr(self, "raw_indexing_results"):
+            self.raw_indexing_results = {}
+        self.raw_indexing_results[phase_id] = (quaternions, metrics)
+        if indices is not None:
+            if not hasattr(self, "raw_indexing_indices"):
+                self.raw_indexing_indices = {}
+            self.raw_indexing_indices[phase_id] = indices
+
+    def combine_indexing_results(self, higher_is_better: bool) -> None:
+        """Keep the best top-1 match over phases (experiment_pats.py:387-451).
+        With several phases the comparison is made on each phase's top-1
+        metric, the evident intent of the reference (whose element-wise
+        comparison of (N, k) against (N,) metrics does not broadcast)."""
+        if not hasattr(self, "raw_indexing_results"):
+            raise ValueError("Raw indexing results must be set before combining.")
+        dev = self.patterns.device
+        res = self.raw_indexing_results
+        if len(res) == 1:
+            pid = next(iter(res))
+            self.phases = torch.full((self.n_patterns,), pid, dtype=torch.uint8, device=dev)
+            self.orientations = res[pid][0][:, 0]
+            self.best_metrics = res[pid][1][:, 0]
+            return
+        best = torch.full((self.n_patterns,), float("-inf") if higher_is_better else float("inf"), device=dev)
+        phases = torch.full((self.n_patterns,), 255, dtype=torch.uint8, device=dev)
+        ori = torch.zeros(self.n_patterns, 4, device=dev)
+        for pid, (q, m) in res.items():
+            m1 = m[:, 0].to(best.dtype)
+            mask = m1 > best if higher_is_better else m1 < best
+            best[mask] = m1[mask]
+            phases[mask] = pid
+            ori[mask] = q[mask, 0].to(ori.dtype)
+        self.phases, self.orientations, self.best_metrics = phases, ori, best

@@ -1,0 +1,190 @@
+"""Dictionary indexing drivers (ebsd/indexing.py:14-211 and :214-476).
+
+dictionary_index_orientations keeps the reference's signature and side
+effects (results written into the ExperimentPatterns object). Underneath,
+the dictionary is projected by the fused HIP projection straight into the
+layout the similarity kernel reads, and every query is scored against it by
+the MFMA similarity kernel with the top-k fused in; the reference's chunk
+sizes are accepted but the engine picks its own (large) chunks, which cannot
+change the result because the top-k order is total (score, then index).
+
+DictionaryIndexer is the reusable engine behind the driver: it holds one
+projected dictionary resident in HBM and indexes batches of patterns
+against it.
+"""
+
+from __future__ import annotations
+
+from typing import List, Optional, Sequence, Union
+
+import torch
+
+from . import kernels
+from ._lib import DTYPE_F32, DTYPE_FP8_E4M3
+from .experiment_pats import ExperimentPatterns
+from .geometry import EBSDGeometry
+from .knn import ChunkedKNN
+from .master_pattern import MasterPattern
+from .orientations import sample_ori_fz_laue_angle
+
+# Dictionary rows projected per launch when the engine streams a dictionary
+# that it does not keep resident.
+_STREAM_ROWS = 1 << 17
+# Keep a projected dictionary resident up to this many bytes.
+_RESIDENT_BYTES = 48 << 30
+
+
+def _gpu_device(t: torch.Tensor) -> torch.device:
+    if t.device.type == "cuda":
+        return t.device
+    if not torch.cuda.is_available():
+        raise RuntimeError("dictionary indexing runs on the GPU and no HIP device is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class DictionaryIndexer:
+    """A projected orientation dictionary resident on one GPU.
+
+    master_pattern (S, 2S) f32, rays (D, 3) f32 unit sample-frame
+    directions, orientations (N, 4) f32. precision selects the similarity
+    arithmetic: "fp32" (the drop-in), "bf16", "fp16" or "fp8"; with
+    rescore_k > 0 the low-precision candidates are re-scored in FP32 against
+    an FP32 copy of the dictionary. dict_unit_norm also divides each
+    dictionary pattern by its norm (cosine ranking); the reference ranks by
+    the raw dot product of zero-mean patterns (False).
+    """
+
+    def __init__(self, master_pattern: torch.Tensor, rays: torch.Tensor, orientations: torch.Tensor,
+                 precision: str = "fp32", rescore_k: int = 0, dict_unit_norm: bool = False,
+                 renorm_dirs: bool = False):
+        if precision not in kernels.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(kernels.PRECISIONS)}")
+        self.device = orientations.device
+        self.mp = master_pattern.to(self.device, torch.float32).contiguous()
+        self.rays = rays.to(self.device, torch.float32).contiguous()
+        self.orientations = orientations.to(self.device, torch.float32).contiguous()
+        self.precision = precision
+        self.dtype = kernels.PRECISIONS[precision]
+        self.rescore_k = int(rescore_k)
+        self.unit_norm = bool(dict_unit_norm)
+        self.renorm_dirs = bool(renorm_dirs)
+        self.n_pix = self.rays.shape[0]
+        self.dictionary = self._project(self.dtype)
+        self.dictionary_f32 = None
+        if self.rescore_k > 0 and self.dtype != DTYPE_F32:
+            self.dictionary_f32 = self._project(DTYPE_F32)
+
+    def _project(self, dtype: int) -> kernels.PatternMatrix:
+        return kernels.project(self.mp, self.rays, self.orientations, dtype, zero_mean=True,
+                               unit_norm=self.unit_norm, renorm_dirs=self.renorm_dirs)
+
+    @property
+    def size(self) -> int:
+        return self.orientations.shape[0]
+
+    def prepare(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None,
+                zero_mean: bool = True, dtype: Optional[int] = None) -> kernels.PatternMatrix:
+        """(N, P) float patterns -> zero-mean padded rows of the search dtype."""
+        return kernels.prep_patterns(patterns, self.dtype if dtype is None else dtype, zero_mean=zero_mean,
+                                     pix_idx=pix_idx)
+
+    def index(self, queries: kernels.PatternMatrix, topk: int, queries_f32: Optional[kernels.PatternMatrix] = None):
+        """Top-k (values (N, k) f32, indices (N, k) int32) of prepared queries."""
+        m = queries.rows
+        k_search = max(topk, self.rescore_k) if self.dictionary_f32 is not None else topk
+        val = torch.empty((m, k_search), dtype=torch.float32, device=self.device)
+        idx = torch.empty((m, k_search), dtype=torch.int32, device=self.device)
+        kernels.sim_topk(queries, self.dictionary, k_search, val, idx, index_base=0, merge=False)
+        if self.dictionary_f32 is not None:
+            if queries_f32 is None:
+                raise ValueError("FP32 rescoring needs the FP32 queries")
+            val, idx = kernels.rescore(queries_f32, self.dictionary_f32, idx, topk)
+        return val, idx
+
+
+def dictionary_index_orientations(
+    master_patterns: Union[MasterPattern, List[MasterPattern]],
+    geometry: EBSDGeometry,
+    experiment_patterns: ExperimentPatterns,
+    signal_mask: Optional[torch.Tensor] = None,
+    subtract_exp_pat_mean: bool = True,
+    experiment_chunk_size: int = 256,
+    dictionary_resolution_degrees: float = 0.5,
+    dictionary_chunk_size: int = 256,
+    virtual_binning: int = 1,
+    top_k_matches: int = 1,
+    distance_metric: str = "angular",
+    match_dtype: torch.dtype = torch.float16,
+    quantized_if_x86: bool = True,
+    average_pattern_center: bool = True,
+    *,
+    dictionary_orientations: Optional[Sequence[torch.Tensor]] = None,
+    precision: Optional[str] = None,
+    rescore_k: int = 0,
+    dict_unit_norm: bool = False,
+    generator: Optional[torch.Generator] = None,
+) -> None:
+    """Index experimental patterns against a projected dictionary per phase
+    (indexing.py:14-211) and store the results in `experiment_patterns`:
+    raw_indexing_results[phase] = (quaternions (N, k, 4), metrics (N, k)),
+    raw_indexing_indices[phase] = dictionary indices (N, k) int64, then
+    phases / orientations / best_metrics via combine_indexing_results.
+
+    Extensions (keyword-only, defaults keep the reference semantics):
+    dictionary_orientations  per-phase (N_dict, 4) dictionaries to use
+                             instead of sampling one
+    precision                "fp32" | "bf16" | "fp16" | "fp8"; default
+                             follows match_dtype
+    rescore_k                re-score this many low-precision candidates in
+                             FP32 (precision form)
+    dict_unit_norm           rank by cosine instead of the raw dot product
+    generator                seeds the dictionary permutation
+    """
+    if isinstance(master_patterns, MasterPattern):
+        master_patterns = [master_patterns]
+    if distance_metric != "angular":
+        raise NotImplementedError(f"'{distance_metric}' not supported.")
+    if not average_pattern_center:
+        raise NotImplementedError("per-scan-point pattern centres are not supported")
+    if precision is None:
+        if match_dtype not in kernels.MATCH_DTYPES:
+            raise NotImplementedError(f"match_dtype {match_dtype} is not supported")
+        precision = {torch.float32: "fp32", torch.bfloat16: "bf16", torch.float16: "fp16"}[match_dtype]
+    n_exp = experiment_patterns.n_patterns
+    for i, mp in enumerate(master_patterns):
+        device = _gpu_device(mp.master_pattern)
+        if dictionary_orientations is not None:
+            ori = dictionary_orientations[i].to(device, torch.float32)
+        else:
+            ori = sample_ori_fz_laue_angle(mp.laue_group, dictionary_resolution_degrees, device, permute=True,
+                                           generator=generator)
+        rays = geometry.get_rays((virtual_binning, virtual_binning))
+        pix_idx = None
+        if signal_mask is not None:
+            flat = signal_mask.flatten().to(rays.device)
+            rays = rays[flat]
+            pix_idx = torch.nonzero(flat.to(device), as_tuple=False).flatten().to(torch.int32)
+        engine = DictionaryIndexer(mp.pattern_for_binning(virtual_binning), rays.to(device), ori,
+                                   precision=precision, rescore_k=rescore_k, dict_unit_norm=dict_unit_norm)
+        exp = experiment_patterns.get_patterns(torch.arange(n_exp, device=experiment_patterns.patterns.device),
+                                               binning=virtual_binning)
+        exp = exp.reshape(n_exp, -1).to(device, torch.float32)
+        # the whole-set query path of the reference honours
+        # subtract_exp_pat_mean; its chunked path always subtracts
+        zero_mean = subtract_exp_pat_mean if experiment_chunk_size > n_exp else True
+        topk = top_k_matches
+        vals, idxs = [], []
+        # bound the prepared query buffer to ~4 GB per step
+        step = max(1, min(n_exp, (4 << 30) // (4 * max(1, exp.shape[1]))))
+        for s in range(0, n_exp, step):
+            q = engine.prepare(exp[s:s + step], pix_idx, zero_mean)
+            qf = engine.prepare(exp[s:s + step], pix_idx, zero_mean, DTYPE_F32) if engine.dictionary_f32 is not None else None
+            v, ix = engine.index(q, topk, qf)
+            vals.append(v)
+            idxs.append(ix)
+        val = torch.cat(vals, 0)
+        idx = torch.cat(idxs, 0).to(torch.int64)
+        if match_dtype in (torch.float16, torch.bfloat16) and precision in ("fp16", "bf16") and rescore_k == 0:
+            val = val.to(match_dtype)
+        experiment_patterns.set_raw_indexing_results(ori[idx.clamp_min(0)], val, phase_id=i, indices=idx)
+    experiment_patterns.combine_indexing_results(higher_is_better=True)

@@ -82,6 +82,17 @@ int ebsd_project(const float* mp, int mp_rows, int mp_cols, const float* rays,
                  void* stream);
 
 /*
+ * Master-pattern lookup at arbitrary directions: out[i] = bilinear sample
+ * (border, align_corners=False) of the side-by-side Lambert master pattern at
+ * direction dirs[i] (re-normalised first when renorm != 0).
+ *   dirs (n, 3) f32; out (n,) f32
+ * Replaces: ebsd/master_pattern.py:141-209 (MasterPattern.interpolate with
+ * mode="bilinear", padding_mode="border", align_corners=False).
+ */
+int ebsd_sample_dirs(const float* mp, int mp_rows, int mp_cols, const float* dirs,
+                     long long n, float* out, int renorm, void* stream);
+
+/*
  * Experimental-pattern preparation: gather pixels (pix_idx may be NULL for
  * the identity), optionally zero-mean / unit-norm each row, convert to
  * out_dtype and zero-pad to ld_out.
@@ -116,6 +127,20 @@ int ebsd_sim_topk(const void* q, const float* q_scale, int m, const void* dict,
                   const float* d_scale, int n, int ld, int dtype, int topk,
                   float* top_val, int* top_idx, int index_base, int merge,
                   void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * FP32 re-scoring of low-precision candidates: for each query row i, the
+ * n_cand dictionary rows cand[i, :] (int32; entries outside [0, n) are
+ * empty slots) are scored exactly in FP32 against q[i, :] and the best
+ * `topk` are written, sorted descending (ties: lower index first).
+ *   q (m, ld) f32; dict (n, ld) f32; cand (m, n_cand) int32
+ *   out_val (m, topk) f32; out_idx (m, topk) int32
+ * No counterpart in the reference: it is the second stage of the precision
+ * form of utils/knn.py:80-103, which restores the FP32 ranking of the
+ * candidates a BF16/FP16/FP8 similarity pass proposes.
+ */
+int ebsd_rescore(const float* q, int m, const float* dict, int n, int ld, const int* cand,
+                 int n_cand, int topk, float* out_val, int* out_idx, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,136 @@
+"""End-to-end dictionary-indexing parity on the GPU against the CPU oracle
+(the reference's FP32 DI restated) and the reference's golden vectors.
+
+Bar: the GPU's top-1 dictionary index equals the oracle's, except where the
+two candidates are tied within 1e-6 in cosine similarity to the query."""
+
+import numpy as np
+import pytest
+import torch
+
+import ebsdtorch_amd as E
+from ebsdtorch_amd import kernels, synthetic
+from oracle import di_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda")
+TIE_COS = 1e-6
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _cos(q, d):
+    q = q.double() - q.double().mean(-1, keepdim=True)
+    return (q * d.double()).sum(-1) / (q.norm(dim=-1) * d.double().norm(dim=-1))
+
+
+def _assert_top1_parity(got_idx, ref_idx, exp, dict_pats):
+    """dict_pats: oracle zero-mean dictionary patterns (N_dict, D)."""
+    diff = torch.nonzero(got_idx != ref_idx).flatten()
+    for r in diff.tolist():
+        cg = _cos(exp[r], dict_pats[got_idx[r]]).item()
+        cr = _cos(exp[r], dict_pats[ref_idx[r]]).item()
+        assert abs(cg - cr) < TIE_COS, (r, got_idx[r].item(), ref_idx[r].item(), cg, cr)
+    return len(diff)
+
+
+@pytest.mark.parametrize("tag,ecs", [("chunked", 5), ("all", 64)])
+def test_driver_matches_reference_golden(golden, tag, ecs):
+    _say(f"golden DI {tag}")
+    mp = E.MasterPattern(torch.tensor(golden["mp33"]).to(DEV), laue_group=11)
+    geom = E.EBSDGeometry((8, 8), proj_center=(0.5, 0.5, 0.6))
+    exp = E.ExperimentPatterns(torch.tensor(golden["di_exp"]).to(DEV))
+    dico = torch.tensor(golden[f"di_{tag}_dict"])
+    E.dictionary_index_orientations(mp, geom, exp, experiment_chunk_size=ecs, dictionary_chunk_size=37,
+                                    top_k_matches=3, match_dtype=torch.float32,
+                                    dictionary_orientations=[dico])
+    q, m = exp.raw_indexing_results[0]
+    ref_q = torch.tensor(golden[f"di_{tag}_quats"])
+    ref_m = torch.tensor(golden[f"di_{tag}_metrics"])
+    idx = exp.raw_indexing_indices[0].cpu()
+    assert torch.allclose(m.cpu(), ref_m, rtol=1e-5, atol=1e-5)
+    # top-1 quaternion identical unless the metric is tied
+    same = (q.cpu()[:, 0] == ref_q[:, 0]).all(-1)
+    assert torch.all(same | ((m.cpu()[:, 0] - m.cpu()[:, 1]).abs() < 1e-6))
+    assert torch.equal(exp.orientations.cpu(), q.cpu()[:, 0])
+    assert exp.phases.dtype == torch.uint8 and torch.all(exp.phases == 0)
+    assert idx.dtype == torch.int64
+
+
+def test_small_config_top1_parity_with_oracle():
+    """Config C1 shape: 201x201 m-3m band MP, ~10k FZ dictionary, 60x60
+    detector at PC (0.5, 0.5, 0.6), 1,000 noisy patterns, top-1."""
+    _say("C1 parity: building inputs")
+    mp = synthetic.band_master_pattern(201, 11, 40, 2.0, seed=0)
+    dico = O.sample_ori_fz_laue(11, 10000)
+    rays = O.normalized_rays((60, 60), (0.5, 0.5, 0.6))
+    ori = synthetic.random_orientations(1000, seed=2)
+    clean = O.project_dictionary(mp, rays, ori) + 0.0  # zero-mean clean
+    g = torch.Generator().manual_seed(3)
+    raw = O.interpolate(mp, O.qu_apply(ori[:, None], rays[None]), normalize_coords=False)
+    counts = torch.poisson(200 * raw / raw.mean(1, keepdim=True), generator=g)
+    exp = counts / 200 + 0.05 * torch.randn(counts.shape, generator=g)
+    _say("C1 parity: oracle DI")
+    ref_idx, ref_val = O.dictionary_index(mp, rays, exp, dico, top_k=1, dictionary_chunk_size=4096,
+                                          experiment_chunk_size=4096)
+    _say("C1 parity: GPU DI")
+    eng = E.DictionaryIndexer(mp.to(DEV), rays.to(DEV), dico.to(DEV))
+    val, idx = eng.index(eng.prepare(exp.to(DEV)), 1)
+    dict_pats = O.project_dictionary(mp, rays, dico, chunk=4096)
+    n_diff = _assert_top1_parity(idx[:, 0].cpu().long(), ref_idx[:, 0], exp, dict_pats)
+    _say(f"C1 parity: {n_diff} tie swaps of 1000")
+    assert torch.allclose(val[:, 0].cpu(), ref_val[:, 0], rtol=1e-4, atol=1e-4)
+    del clean
+
+
+def test_driver_with_signal_mask_and_binning():
+    _say("mask + binning")
+    mp = synthetic.band_master_pattern(101, 11, 20, 3.0, seed=1)
+    geom = E.EBSDGeometry((20, 20), proj_center=(0.5, 0.5, 0.6))
+    dico = O.sample_ori_fz_laue(11, 3000)
+    ori = synthetic.random_orientations(64, seed=5)
+    rays_full = O.normalized_rays((20, 20), (0.5, 0.5, 0.6))
+    pats = O.interpolate(mp, O.qu_apply(ori[:, None], rays_full[None]), normalize_coords=False).view(64, 20, 20)
+    mask = torch.zeros(10, 10, dtype=torch.bool)
+    mask[1:9, 2:9] = True
+    exp = E.ExperimentPatterns(pats.to(DEV))
+    E.dictionary_index_orientations(E.MasterPattern(mp.to(DEV), laue_group=11), geom, exp, signal_mask=mask,
+                                    virtual_binning=2, top_k_matches=2, match_dtype=torch.float32,
+                                    dictionary_orientations=[dico])
+    # oracle: binned MP, binned geometry, binned patterns, masked
+    mpb = torch.nn.functional.avg_pool2d(mp[None, None], 2, stride=2)[0, 0]
+    rays_b = O.normalized_rays((20, 20), (0.5, 0.5, 0.6), binning=(2, 2))
+    pb = torch.nn.functional.avg_pool2d(pats, 2).reshape(64, -1)
+    ref_idx, _ = O.dictionary_index(mpb, rays_b, pb, dico, top_k=2, signal_mask=mask,
+                                    dictionary_chunk_size=1000, experiment_chunk_size=1000)
+    dict_pats = O.project_dictionary(mpb, rays_b, dico, chunk=1000, signal_mask=mask)
+    q = pb[:, mask.flatten()]
+    _assert_top1_parity(exp.raw_indexing_indices[0][:, 0].cpu(), ref_idx[:, 0], q, dict_pats)
+
+
+def test_full_size_self_retrieval_333k():
+    """Config C2 dictionary size (333k m-3m orientations, 501x501 MP, 60x60):
+    noise-free patterns simulated at dictionary orientations must retrieve
+    their own entry (size-independent property of the full-size path)."""
+    _say("C2 self retrieval: dictionary")
+    mp = synthetic.band_master_pattern(501, 11, 40, 2.0, seed=0, device=DEV)
+    dico = E.sample_ori_fz_laue(11, 333_000, DEV, permute=True, generator=torch.Generator().manual_seed(1))
+    assert 325_000 < dico.shape[0] < 335_000
+    rays = E.EBSDGeometry((60, 60), proj_center=(0.5, 0.5, 0.6)).get_rays().to(DEV)
+    eng = E.DictionaryIndexer(mp, rays, dico)
+    pick = torch.randperm(dico.shape[0], generator=torch.Generator().manual_seed(7))[:4096].to(DEV)
+    raw = kernels.project(mp, rays, dico[pick], zero_mean=False).data[:, :3600]
+    _say("C2 self retrieval: index")
+    val, idx = eng.index(eng.prepare(raw), 16)
+    top1 = idx[:, 0].long()
+    hit = top1 == pick
+    if not torch.all(hit):
+        # only exact duplicates (symmetry-equivalent boundary points) may win
+        d = eng.dictionary.to_float()
+        bad = torch.nonzero(~hit).flatten()
+        a, b = d[pick[bad]].double(), d[top1[bad]].double()
+        cos = (a * b).sum(-1) / (a.norm(dim=-1) * b.norm(dim=-1))
+        assert torch.all(cos > 1 - 1e-6)
+    assert torch.all(val[:, 1:] <= val[:, :-1])

@@ -1,0 +1,190 @@
+"""GPU parity of each kernel against the CPU oracle (or a float64 brute
+force), called through the C ABI."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import ebsdtorch_amd as E
+from ebsdtorch_amd import kernels, synthetic
+from ebsdtorch_amd._lib import DTYPE_BF16, DTYPE_F16, DTYPE_F32, DTYPE_FP8_E4M3
+from oracle import di_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda")
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def mp101():
+    return synthetic.band_master_pattern(101, 11, 40, 2.0, seed=0)
+
+
+def _rays(shape=(60, 60), pc=(0.5, 0.5, 0.6)):
+    return O.normalized_rays(shape, pc)
+
+
+def test_sample_dirs_matches_oracle_interpolate(golden):
+    _say("sample_dirs vs golden interpolate (33x66 MP, 3000 points)")
+    mp = E.MasterPattern(torch.tensor(golden["mp33"]).to(DEV), laue_group=11)
+    pts = torch.tensor(golden["interp_pts"])
+    got = mp.interpolate(pts.to(DEV), normalize_coords=False).cpu()
+    ref = torch.tensor(golden["interp_out"])
+    assert (got - ref).abs().max().item() < 2e-5
+
+
+@pytest.mark.parametrize("shape,n", [((60, 60), 300), ((8, 8), 50), ((80, 80), 40), ((120, 120), 12)])
+def test_projection_matches_oracle(mp101, shape, n):
+    _say(f"projection {shape} n={n}")
+    rays = _rays(shape)
+    q = synthetic.random_orientations(n, seed=11)
+    ref = O.project_dictionary(mp101, rays, q)  # zero-mean, reference op order
+    got = kernels.project(mp101.to(DEV), rays.to(DEV), q.to(DEV))
+    assert got.ld % 32 == 0 and got.ld >= rays.shape[0]
+    g = got.data.cpu()
+    assert torch.all(g[:, rays.shape[0]:] == 0)
+    assert (g[:, : rays.shape[0]] - ref).abs().max().item() < 2e-5
+
+
+def test_projection_raw_unit_norm_and_renorm(mp101):
+    _say("projection flags")
+    rays = _rays()
+    coords = O.detector_coords((60, 60), proj_center=(0.5, 0.5, 0.6))
+    q = synthetic.random_orientations(64, seed=12)
+    raw = kernels.project(mp101.to(DEV), rays.to(DEV), q.to(DEV), zero_mean=False).to_float().cpu()
+    ref_raw = O.interpolate(mp101, O.qu_apply(q[:, None], rays[None]), normalize_coords=False)
+    assert (raw - ref_raw).abs().max().item() < 2e-5
+    # normalize_coords=True path of PCA DI: un-normalised coordinates
+    rn = kernels.project(mp101.to(DEV), coords.to(DEV), q.to(DEV), zero_mean=False, renorm_dirs=True).to_float().cpu()
+    ref_rn = O.interpolate(mp101, O.qu_apply(q[:, None], coords[None]), normalize_coords=True)
+    assert (rn - ref_rn).abs().max().item() < 2e-5
+    un = kernels.project(mp101.to(DEV), rays.to(DEV), q.to(DEV), unit_norm=True).to_float().cpu().double()
+    zm = ref_raw.double() - ref_raw.double().mean(1, keepdim=True)
+    assert torch.allclose(un, zm / zm.norm(dim=1, keepdim=True), atol=1e-6)
+
+
+@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_F16, DTYPE_FP8_E4M3])
+def test_projection_low_precision_outputs(mp101, dtype):
+    _say(f"projection dtype {dtype}")
+    rays = _rays()
+    q = synthetic.random_orientations(100, seed=13)
+    f32 = kernels.project(mp101.to(DEV), rays.to(DEV), q.to(DEV)).to_float()
+    lp = kernels.project(mp101.to(DEV), rays.to(DEV), q.to(DEV), dtype)
+    assert lp.ld % kernels.pad_multiple(dtype) == 0
+    if dtype == DTYPE_FP8_E4M3:
+        # stored bytes are the OCP e4m3 rounding of value / scale
+        scale = f32.abs().amax(1) / 448.0
+        assert torch.allclose(lp.scale, scale, rtol=1e-6)
+        expect = (f32 * (1.0 / scale)[:, None]).to(torch.float8_e4m3fn).view(torch.uint8)
+        assert torch.equal(lp.data[:, :3600], expect)
+        assert torch.all(lp.data[:, 3600:] == 0)
+    else:
+        tdt = torch.bfloat16 if dtype == DTYPE_BF16 else torch.float16
+        assert torch.equal(lp.data[:, :3600], f32.to(tdt))
+
+
+def test_prep_patterns_mean_mask_and_norm():
+    _say("prep patterns")
+    g = torch.Generator().manual_seed(4)
+    x = torch.rand(37, 3600, generator=g) * 3 + 1
+    got = kernels.prep_patterns(x.to(DEV)).to_float().cpu()
+    assert torch.allclose(got, x - x.mean(1, keepdim=True), atol=2e-6)
+    idx = torch.nonzero(torch.rand(3600, generator=g) > 0.3).flatten()
+    got = kernels.prep_patterns(x.to(DEV), pix_idx=idx.to(DEV)).to_float().cpu()
+    xs = x[:, idx]
+    assert got.shape[1] == len(idx)
+    assert torch.allclose(got, xs - xs.mean(1, keepdim=True), atol=2e-6)
+    got = kernels.prep_patterns(x.to(DEV), unit_norm=True).to_float().cpu().double()
+    z = x.double() - x.double().mean(1, keepdim=True)
+    assert torch.allclose(got, z / z.norm(dim=1, keepdim=True), atol=1e-6)
+
+
+def _check_topk(val, idx, q, d, k, tol=1e-4):
+    """val/idx against float64 brute force, allowing swaps only between
+    near-equal scores."""
+    full = q.double() @ d.double().T
+    ref_v, ref_i = torch.topk(full, k, dim=1)
+    assert torch.allclose(val.double(), ref_v, atol=tol * ref_v.abs().max().item())
+    mism = idx != ref_i
+    if mism.any():
+        gv = full.gather(1, idx.clamp_min(0).long())
+        assert torch.allclose(gv[mism], ref_v[mism], atol=tol * ref_v.abs().max().item())
+    # rows sorted descending
+    assert torch.all(val[:, 1:] <= val[:, :-1])
+
+
+@pytest.mark.parametrize("m,n,d,k", [(300, 5000, 200, 1), (300, 5000, 200, 4), (129, 1000, 3600, 16),
+                                     (1, 130, 64, 16), (1000, 20000, 96, 8)])
+def test_sim_topk_f32_brute_force(m, n, d, k):
+    _say(f"sim_topk f32 m={m} n={n} d={d} k={k}")
+    g = torch.Generator().manual_seed(m + n + d + k)
+    q = torch.randn(m, d, generator=g)
+    dd = torch.randn(n, d, generator=g)
+    Q = kernels.prep_patterns(q.to(DEV), zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), zero_mean=False)
+    val = torch.empty(m, k, device=DEV)
+    idx = torch.empty(m, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, val, idx, merge=False)
+    _check_topk(val.cpu(), idx.cpu(), q, dd, k)
+
+
+def test_sim_topk_streaming_merge_equals_one_pass():
+    _say("sim_topk streaming merge")
+    g = torch.Generator().manual_seed(5)
+    q = torch.randn(200, 128, generator=g)
+    dd = torch.randn(7000, 128, generator=g)
+    Q = kernels.prep_patterns(q.to(DEV), zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), zero_mean=False)
+    k = 16
+    v1 = torch.empty(200, k, device=DEV)
+    i1 = torch.empty(200, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, v1, i1, merge=False)
+    v2 = torch.full((200, k), float("-inf"), device=DEV)
+    i2 = torch.full((200, k), -1, dtype=torch.int32, device=DEV)
+    for s in range(0, 7000, 999):
+        kernels.sim_topk(Q, D.narrow(0, s, min(999, 7000 - s)), k, v2, i2, index_base=s, merge=True)
+    assert torch.equal(i1, i2) and torch.equal(v1, v2)
+    # the ChunkedKNN wrapper: chunked data and queries, global indices
+    kn = E.ChunkedKNN(7000, 200, 5, DEV, match_dtype=torch.float32)
+    for s in range(0, 7000, 1500):
+        kn.set_data_chunk(dd[s:s + 1500].to(DEV))
+        for qs in range(0, 200, 64):
+            kn.query_chunk(q[qs:qs + 64].to(DEV), qs)
+    ki, kv = kn.retrieve_topk()
+    assert ki.dtype == torch.int64
+    assert torch.equal(ki.cpu(), i1[:, :5].long().cpu())
+
+
+def test_sim_topk_fewer_rows_than_k_and_ties():
+    _say("sim_topk ties")
+    q = torch.ones(3, 32)
+    dd = torch.ones(5, 32)  # all scores equal: lower index wins
+    Q = kernels.prep_patterns(q.to(DEV), zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), zero_mean=False)
+    val = torch.empty(3, 8, device=DEV)
+    idx = torch.empty(3, 8, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, 8, val, idx, merge=False)
+    assert idx.cpu().tolist() == [[0, 1, 2, 3, 4, -1, -1, -1]] * 3
+    assert torch.all(val[:, :5] == 32) and torch.all(torch.isinf(val[:, 5:]))
+
+
+def test_rescore_exact_fp32():
+    _say("rescore")
+    g = torch.Generator().manual_seed(8)
+    q = torch.randn(100, 300, generator=g)
+    dd = torch.randn(900, 300, generator=g)
+    Q = kernels.prep_patterns(q.to(DEV), zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), zero_mean=False)
+    cand = torch.randint(0, 900, (100, 16), generator=g, dtype=torch.int32)
+    cand[:, -1] = -1
+    v, i = kernels.rescore(Q, D, cand.to(DEV), 4)
+    full = (q.double() @ dd.double().T).gather(1, cand.clamp_min(0).long())
+    full[:, -1] = -math.inf
+    rv, ri = torch.topk(full, 4, dim=1)
+    assert torch.allclose(v.cpu().double(), rv, atol=1e-4)
+    assert torch.equal(i.cpu().long(), cand.long().gather(1, ri))
