@@ -86,6 +86,16 @@ __device__ __forceinline__ float sample_border(const float* __restrict__ img, in
   return ((v_nw * (s * e) + v_ne * (s * w)) + v_sw * (n * e)) + v_se * (n * w);
 }
 
+// x / |x| with the squared norm accumulated as fma(z, z, fma(y, y, x*x)),
+// the rounding sequence of torch's float32 CPU vector norm over a length-3
+// row (master_pattern.py:166-167).
+__device__ __forceinline__ void renormalise(float& x, float& y, float& z) {
+  const float nr = sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
+  x = x / nr;
+  y = y / nr;
+  z = z / nr;
+}
+
 template <typename T>
 __device__ __forceinline__ T to_out(float v);
 template <>
@@ -153,13 +163,12 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
 
 template <int ITEMS, typename OutT>
 __global__ __launch_bounds__(256) void project_kernel(
-    const float* __restrict__ mp, int S, const float* __restrict__ rays, int n_pix,
+    const float* __restrict__ mp, int H, int W, const float* __restrict__ rays, int n_pix,
     const float* __restrict__ quats, OutT* __restrict__ out, int ld,
     float* __restrict__ out_scale, int flags) {
   __shared__ float red[4];
   const size_t n = blockIdx.x;
   const float4 q = *reinterpret_cast<const float4*>(quats + 4 * n);
-  const int W = 2 * S;
   const bool renorm = (flags & EBSD_FLAG_RENORM_DIRS) != 0;
   float vals[ITEMS];
 #pragma unroll
@@ -172,15 +181,10 @@ __global__ __launch_bounds__(256) void project_kernel(
       const float bz = __ldg(rays + 3 * d + 2);
       float x, y, z;
       rotate_ray(q.x, q.y, q.z, q.w, bx, by, bz, x, y, z);
-      if (renorm) {
-        const float nr = sqrtf((x * x + y * y) + z * z);
-        x = x / nr;
-        y = y / nr;
-        z = z / nr;
-      }
+      if (renorm) renormalise(x, y, z);
       float gx, gy;
       lambert_sbs(x, y, z, gx, gy);
-      v = sample_border(mp, S, W, gx, gy);
+      v = sample_border(mp, H, W, gx, gy);
     }
     vals[i] = v;
   }
@@ -206,32 +210,27 @@ __global__ __launch_bounds__(256) void prep_kernel(
 }
 
 // Sample the master pattern at arbitrary sphere directions (one per thread).
-__global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restrict__ mp, int S,
+__global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restrict__ mp, int H, int W,
                                                           const float* __restrict__ dirs, int64_t n,
                                                           float* __restrict__ out, int renorm) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
-  if (renorm) {
-    const float nr = sqrtf((x * x + y * y) + z * z);
-    x = x / nr;
-    y = y / nr;
-    z = z / nr;
-  }
+  if (renorm) renormalise(x, y, z);
   float gx, gy;
   lambert_sbs(x, y, z, gx, gy);
-  out[i] = sample_border(mp, S, 2 * S, gx, gy);
+  out[i] = sample_border(mp, H, W, gx, gy);
 }
 
 template <typename OutT>
-int launch_project(const float* mp, int S, const float* rays, int n_pix, const float* quats,
+int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, const float* quats,
                    int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st) {
   const dim3 grid(n_ori), block(256);
   OutT* o = static_cast<OutT*>(out);
   const int items = (n_pix + 255) / 256;
 #define EBSD_PROJ_CASE(I)                                                                        \
   if (items <= I) {                                                                              \
-    hipLaunchKernelGGL((project_kernel<I, OutT>), grid, block, 0, st, mp, S, rays, n_pix, quats, \
+    hipLaunchKernelGGL((project_kernel<I, OutT>), grid, block, 0, st, mp, H, W, rays, n_pix, quats, \
                        o, ld, out_scale, flags);                                                 \
     EBSD_LAUNCH_CHECK();                                                                         \
     return EBSD_OK;                                                                              \
@@ -291,7 +290,7 @@ extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const flo
                             int n_pix, const float* quats, int n_ori, void* out, int out_dtype,
                             int ld_out, float* out_scale, int flags, void* stream) {
   if (!mp || !rays || !quats || !out) return EBSD_ERR_NULL;
-  if (mp_rows < 1 || mp_cols != 2 * mp_rows || n_ori < 0) return EBSD_ERR_SHAPE;
+  if (mp_rows < 1 || mp_cols < 1 || n_ori < 0) return EBSD_ERR_SHAPE;
   const int chk = check_ld(out_dtype, n_pix, ld_out);
   if (chk) return chk;
   if (out_dtype == EBSD_DTYPE_FP8_E4M3 && !out_scale) return EBSD_ERR_NULL;
@@ -300,13 +299,13 @@ extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const flo
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (out_dtype) {
     case EBSD_DTYPE_F32:
-      return launch_project<float>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+      return launch_project<float>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
     case EBSD_DTYPE_BF16:
-      return launch_project<__hip_bfloat16>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+      return launch_project<__hip_bfloat16>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
     case EBSD_DTYPE_F16:
-      return launch_project<__half>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+      return launch_project<__half>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
     case EBSD_DTYPE_FP8_E4M3:
-      return launch_project<uint8_t>(mp, mp_rows, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+      return launch_project<uint8_t>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
   }
   return EBSD_ERR_DTYPE;
 }
@@ -314,12 +313,12 @@ extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const flo
 extern "C" int ebsd_sample_dirs(const float* mp, int mp_rows, int mp_cols, const float* dirs,
                                 long long n, float* out, int renorm, void* stream) {
   if (!mp || !dirs || !out) return EBSD_ERR_NULL;
-  if (mp_rows < 1 || mp_cols != 2 * mp_rows || n < 0) return EBSD_ERR_SHAPE;
+  if (mp_rows < 1 || mp_cols < 1 || n < 0) return EBSD_ERR_SHAPE;
   if (n == 0) return EBSD_OK;
   const int64_t blocks = (n + 255) / 256;
   if (blocks > 0x7fffffffLL) return EBSD_ERR_SHAPE;
   hipLaunchKernelGGL(sample_dirs_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), mp, mp_rows, dirs, (int64_t)n, out, renorm);
+                     static_cast<hipStream_t>(stream), mp, mp_rows, mp_cols, dirs, (int64_t)n, out, renorm);
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;
 }

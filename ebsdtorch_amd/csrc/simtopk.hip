@@ -4,6 +4,13 @@
 // "angular") are produced tile by tile on the matrix cores and reduced at once
 // to a running top-k per query held in VGPRs; S never reaches memory.
 //
+// Element types: FP32 (v_mfma_f32_32x32x2_f32), BF16 / FP16
+// (v_mfma_f32_32x32x16_{bf16,f16}) and OCP FP8 e4m3 (v_mfma_scale_f32_
+// 32x32x64_f8f6f4 with unit block scales; FP8 scores are multiplied by the
+// per-row query and dictionary scales in the epilogue). All accumulate in
+// FP32. Data movement is byte based and identical for every type: a stage is
+// 128 bytes of K per row (32 FP32, 64 BF16/FP16 or 128 FP8 elements).
+//
 // Work decomposition
 //   grid.x  : query tiles of 128 rows
 //   grid.y  : dictionary splits; split s walks dictionary tiles
@@ -11,7 +18,7 @@
 //   block   : 256 threads = 4 waves; wave w owns queries [32 w, 32 w + 32)
 //             of the block and all 128 dictionary rows of the current tile,
 //             i.e. a 128 x 32 score block = 4 MFMA 32x32 tiles (64 acc VGPRs)
-// The K dimension streams through double-buffered LDS in 32-float stages;
+// The K dimension streams through double-buffered LDS in 128-byte stages;
 // the next stage is fetched into registers while the current one feeds the
 // MFMAs. After each dictionary tile, every lane folds its 64 fresh scores
 // (1 query column x 64 dictionary rows) into one register top-k list, so a
@@ -26,11 +33,19 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
 
-constexpr int TILE = 128;        // dictionary rows per tile, queries per block
-constexpr int BK32 = 32;         // f32 elements of K per LDS stage
-constexpr int LDS_STRIDE = 36;   // padded row stride (floats): conflict-free b128 reads
-constexpr int LDS_TILE = TILE * LDS_STRIDE;
+constexpr int TILE = 128;          // dictionary rows per tile, queries per block
+constexpr int STAGE_BYTES = 128;   // bytes of K per row per LDS stage
+constexpr int LDS_ROW = 144;       // padded LDS row stride (bytes): conflict-free b128 reads
+constexpr int LDS_TILE = TILE * LDS_ROW;
+constexpr int LDS_BYTES = 4 * LDS_TILE;  // 2 stages x (dictionary tile + query tile)
+
+// E8M0 exponent of 1.0: unit block scales for the MX-scaled FP8 MFMA.
+constexpr int kUnitScale = 127;
 
 // Per-lane position of accumulator element r of a 32x32 MFMA tile:
 // column = lane % 32, row = 8 * (r / 4) + 4 * (lane / 32) + r % 4.
@@ -38,11 +53,68 @@ __device__ __forceinline__ int mfma32_row(int r, int lane) {
   return ((r >> 2) << 3) + ((lane >> 5) << 2) + (r & 3);
 }
 
-template <int K>
-__global__ __launch_bounds__(256, 2) void sim_topk_f32_kernel(
-    const float* __restrict__ Q, int M, const float* __restrict__ Dm, int N, int ld,
+// One LDS stage of the K loop for a 128 (dictionary) x 32 (query) wave tile.
+// Lane half h = lane / 32 consumes bytes [64 h, 64 h + 64) of each row's
+// 128-byte stage in 16-byte reads; A (dictionary) and B (query) use the same
+// K order, so every MFMA sums matching products whatever the per-instruction
+// K assignment of the hardware.
+template <int DT>
+__device__ __forceinline__ void stage_mfma(const unsigned char* As, const unsigned char* Bs, int a_off,
+                                           int b_off, floatx16 (&acc)[4]) {
+  if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const intx4 b0 = *reinterpret_cast<const intx4*>(Bs + b_off + s * 32);
+      const intx4 b1 = *reinterpret_cast<const intx4*>(Bs + b_off + s * 32 + 16);
+      const intx8 b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi) {
+        const unsigned char* ap = As + a_off + bi * 32 * LDS_ROW + s * 32;
+        const intx4 a0 = *reinterpret_cast<const intx4*>(ap);
+        const intx4 a1 = *reinterpret_cast<const intx4*>(ap + 16);
+        const intx8 a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+        // cbsz = blgp = 0: both operands OCP e4m3
+        acc[bi] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, acc[bi], 0, 0, 0, kUnitScale, 0,
+                                                                  kUnitScale);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const floatx4 braw = *reinterpret_cast<const floatx4*>(Bs + b_off + s * 16);
+      floatx4 araw[4];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+        araw[bi] = *reinterpret_cast<const floatx4*>(As + a_off + bi * 32 * LDS_ROW + s * 16);
+      if constexpr (DT == EBSD_DTYPE_F32) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi)
+            acc[bi] = __builtin_amdgcn_mfma_f32_32x32x2f32(araw[bi][e], braw[e], acc[bi], 0, 0, 0);
+      } else if constexpr (DT == EBSD_DTYPE_BF16) {
+        const bf16x8 b = __builtin_bit_cast(bf16x8, braw);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+          acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, araw[bi]), b, acc[bi],
+                                                            0, 0, 0);
+      } else {
+        const halfx8 b = __builtin_bit_cast(halfx8, braw);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+          acc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(halfx8, araw[bi]), b, acc[bi],
+                                                           0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int DT, int K>
+__global__ __launch_bounds__(256, 2) void sim_topk_kernel(
+    const unsigned char* __restrict__ Q, const float* __restrict__ q_scale, int M,
+    const unsigned char* __restrict__ Dm, const float* __restrict__ d_scale, int N, int ld_bytes,
     int tiles_per_split, float* __restrict__ pval, int* __restrict__ pidx, int index_base) {
-  __shared__ __attribute__((aligned(16))) float smem[4 * LDS_TILE];
+  __shared__ __attribute__((aligned(16))) unsigned char smem[LDS_BYTES];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -52,31 +124,32 @@ __global__ __launch_bounds__(256, 2) void sim_topk_f32_kernel(
   const int t_begin = split * tiles_per_split;
   const int t_end = min(t_begin + tiles_per_split, n_tiles);
 
-  // Global->LDS mapping: thread loads rows lr + 32 i (i < 4), floats lc..lc+3
-  // of both the dictionary tile and the query tile.
+  // Global->LDS mapping: thread copies 16 bytes at byte lc of rows lr + 32 i
+  // (i < 4) of both the dictionary tile and the query tile.
   const int lr = tid >> 3;
-  const int lc = (tid & 7) * 4;
-  const float* qrow[4];
+  const int lc = (tid & 7) * 16;
+  const unsigned char* qrow[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) qrow[i] = Q + (size_t)min(q0 + lr + 32 * i, M - 1) * ld + lc;
+  for (int i = 0; i < 4; ++i) qrow[i] = Q + (size_t)min(q0 + lr + 32 * i, M - 1) * ld_bytes + lc;
+
+  // FP8: this lane's query scale (its query column is fixed)
+  float qs = 1.0f;
+  if constexpr (DT == EBSD_DTYPE_FP8_E4M3) qs = q_scale[min(q0 + wave * 32 + (lane & 31), M - 1)];
 
   float tv[K];
   int ti[K];
   topk_init<K>(tv, ti);
 
-  const int nk = ld / BK32;
-  // LDS read offsets (floats) of this lane inside a stage. Lane half h = lane/32
-  // supplies K elements [16 h, 16 h + 16) of the stage, four per b128 read;
-  // A and B use the same K order, so each MFMA sums matching products.
-  const int a_off = (lane & 31) * LDS_STRIDE + (lane >> 5) * 16;
-  const int b_off = (wave * 32 + (lane & 31)) * LDS_STRIDE + (lane >> 5) * 16;
-  const int st_off = lr * LDS_STRIDE + lc;
+  const int nk = ld_bytes / STAGE_BYTES;
+  const int a_off = (lane & 31) * LDS_ROW + (lane >> 5) * 64;
+  const int b_off = (wave * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 64;
+  const int st_off = lr * LDS_ROW + lc;
 
   for (int t = t_begin; t < t_end; ++t) {
     const int d0 = t * TILE;
-    const float* drow[4];
+    const unsigned char* drow[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) drow[i] = Dm + (size_t)min(d0 + lr + 32 * i, N - 1) * ld + lc;
+    for (int i = 0; i < 4; ++i) drow[i] = Dm + (size_t)min(d0 + lr + 32 * i, N - 1) * ld_bytes + lc;
 
     floatx16 acc[4];
 #pragma unroll
@@ -92,8 +165,8 @@ __global__ __launch_bounds__(256, 2) void sim_topk_f32_kernel(
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      *reinterpret_cast<floatx4*>(smem + st_off + 32 * i * LDS_STRIDE) = ra[i];
-      *reinterpret_cast<floatx4*>(smem + LDS_TILE + st_off + 32 * i * LDS_STRIDE) = rb[i];
+      *reinterpret_cast<floatx4*>(smem + st_off + 32 * i * LDS_ROW) = ra[i];
+      *reinterpret_cast<floatx4*>(smem + LDS_TILE + st_off + 32 * i * LDS_ROW) = rb[i];
     }
     __syncthreads();
 
@@ -101,34 +174,21 @@ __global__ __launch_bounds__(256, 2) void sim_topk_f32_kernel(
       const int cur = kt & 1;
       const bool more = (kt + 1) < nk;
       if (more) {
-        const int ko = (kt + 1) * BK32;
+        const int ko = (kt + 1) * STAGE_BYTES;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           ra[i] = *reinterpret_cast<const floatx4*>(drow[i] + ko);
           rb[i] = *reinterpret_cast<const floatx4*>(qrow[i] + ko);
         }
       }
-      const float* As = smem + cur * 2 * LDS_TILE;
-      const float* Bs = As + LDS_TILE;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const floatx4 b = *reinterpret_cast<const floatx4*>(Bs + b_off + s * 4);
-        floatx4 a[4];
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi)
-          a[bi] = *reinterpret_cast<const floatx4*>(As + a_off + bi * 32 * LDS_STRIDE + s * 4);
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-#pragma unroll
-          for (int bi = 0; bi < 4; ++bi)
-            acc[bi] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[bi][e], b[e], acc[bi], 0, 0, 0);
-      }
+      const unsigned char* As = smem + cur * 2 * LDS_TILE;
+      stage_mfma<DT>(As, As + LDS_TILE, a_off, b_off, acc);
       if (more) {
-        float* Ad = smem + (cur ^ 1) * 2 * LDS_TILE;
+        unsigned char* Ad = smem + (cur ^ 1) * 2 * LDS_TILE;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          *reinterpret_cast<floatx4*>(Ad + st_off + 32 * i * LDS_STRIDE) = ra[i];
-          *reinterpret_cast<floatx4*>(Ad + LDS_TILE + st_off + 32 * i * LDS_STRIDE) = rb[i];
+          *reinterpret_cast<floatx4*>(Ad + st_off + 32 * i * LDS_ROW) = ra[i];
+          *reinterpret_cast<floatx4*>(Ad + LDS_TILE + st_off + 32 * i * LDS_ROW) = rb[i];
         }
       }
       // Orders this stage's reads before the next overwrite of its buffer,
@@ -143,14 +203,18 @@ __global__ __launch_bounds__(256, 2) void sim_topk_f32_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int d = d0 + bi * 32 + mfma32_row(r, lane);
-        if (d < N) topk_offer<K>(tv, ti, acc[bi][r], index_base + d);
+        if (d < N) {
+          float v = acc[bi][r];
+          if constexpr (DT == EBSD_DTYPE_FP8_E4M3) v = v * (qs * d_scale[d]);
+          topk_offer<K>(tv, ti, v, index_base + d);
+        }
       }
     }
   }
 
   // Merge the 2 lists per query (the two lane halves of its wave) in LDS.
-  float* cv = smem;  // [128 queries][2 lists][K]
-  int* ci = reinterpret_cast<int*>(smem + TILE * 2 * K);
+  float* cv = reinterpret_cast<float*>(smem);  // [128 queries][2 lists][K]
+  int* ci = reinterpret_cast<int*>(smem) + TILE * 2 * K;
   {
     const int ql = wave * 32 + (lane & 31);
     const int list = lane >> 5;
@@ -297,13 +361,48 @@ extern "C" int ebsd_rescore(const float* q, int m, const float* dict, int n, int
   return EBSD_OK;
 }
 
+static int elem_bytes(int dtype) {
+  switch (dtype) {
+    case EBSD_DTYPE_F32: return 4;
+    case EBSD_DTYPE_BF16:
+    case EBSD_DTYPE_F16: return 2;
+    case EBSD_DTYPE_FP8_E4M3: return 1;
+  }
+  return 0;
+}
+
 extern "C" int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t* bytes) {
   if (!bytes) return EBSD_ERR_NULL;
   if (m < 0 || n < 1) return EBSD_ERR_SHAPE;
   if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
-  if (dtype != EBSD_DTYPE_F32) return EBSD_ERR_DTYPE;
+  if (elem_bytes(dtype) == 0) return EBSD_ERR_DTYPE;
   const SimPlan p = plan_sim(m, n, topk);
   *bytes = workspace_bytes(p, m);
+  return EBSD_OK;
+}
+
+template <int DT>
+static int launch_sim(const SimPlan& p, const void* q, const float* q_scale, int m, const void* dict,
+                      const float* d_scale, int n, int ld_bytes, int topk, float* top_val, int* top_idx,
+                      int index_base, int merge, float* pval, int* pidx, hipStream_t st) {
+  const dim3 grid(p.n_qtiles, p.n_splits), block(256);
+  const dim3 mgrid((m + 255) / 256);
+  const unsigned char* Qb = static_cast<const unsigned char*>(q);
+  const unsigned char* Db = static_cast<const unsigned char*>(dict);
+  if (p.k_list == 1) {
+    hipLaunchKernelGGL((sim_topk_kernel<DT, 1>), grid, block, 0, st, Qb, q_scale, m, Db, d_scale, n, ld_bytes,
+                       p.tiles_per_split, pval, pidx, index_base);
+    EBSD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(topk_merge_kernel<1>, mgrid, dim3(256), 0, st, pval, pidx, p.n_splits, m, topk, top_val,
+                       top_idx, merge);
+  } else {
+    hipLaunchKernelGGL((sim_topk_kernel<DT, EBSD_MAX_TOPK>), grid, block, 0, st, Qb, q_scale, m, Db, d_scale, n,
+                       ld_bytes, p.tiles_per_split, pval, pidx, index_base);
+    EBSD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(topk_merge_kernel<EBSD_MAX_TOPK>, mgrid, dim3(256), 0, st, pval, pidx, p.n_splits, m,
+                       topk, top_val, top_idx, merge);
+  }
+  EBSD_LAUNCH_CHECK();
   return EBSD_OK;
 }
 
@@ -311,14 +410,14 @@ extern "C" int ebsd_sim_topk(const void* q, const float* q_scale, int m, const v
                              const float* d_scale, int n, int ld, int dtype, int topk,
                              float* top_val, int* top_idx, int index_base, int merge,
                              void* workspace, size_t ws_bytes, void* stream) {
-  (void)q_scale;
-  (void)d_scale;
   if (!q || !dict || !top_val || !top_idx) return EBSD_ERR_NULL;
   if (m < 0 || n < 1 || ld < 1) return EBSD_ERR_SHAPE;
   if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
-  if (dtype != EBSD_DTYPE_F32) return EBSD_ERR_DTYPE;
+  const int eb = elem_bytes(dtype);
+  if (eb == 0) return EBSD_ERR_DTYPE;
+  if (dtype == EBSD_DTYPE_FP8_E4M3 && (!q_scale || !d_scale)) return EBSD_ERR_NULL;
   if (ld % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
-  if ((int64_t)index_base + n > (int64_t)INT_MAX) return EBSD_ERR_SHAPE;
+  if ((int64_t)index_base + n > (int64_t)INT_MAX || index_base < 0) return EBSD_ERR_SHAPE;
   if (((reinterpret_cast<uintptr_t>(q) | reinterpret_cast<uintptr_t>(dict)) & 15) != 0)
     return EBSD_ERR_ALIGN;
   if (m == 0) return EBSD_OK;
@@ -327,23 +426,20 @@ extern "C" int ebsd_sim_topk(const void* q, const float* q_scale, int m, const v
   float* pval = static_cast<float*>(workspace);
   int* pidx = reinterpret_cast<int*>(pval + (size_t)p.n_splits * p.k_list * m);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 grid(p.n_qtiles, p.n_splits), block(256);
-  const float* Qf = static_cast<const float*>(q);
-  const float* Df = static_cast<const float*>(dict);
-  const dim3 mgrid((m + 255) / 256);
-  if (p.k_list == 1) {
-    hipLaunchKernelGGL(sim_topk_f32_kernel<1>, grid, block, 0, st, Qf, m, Df, n, ld,
-                       p.tiles_per_split, pval, pidx, index_base);
-    EBSD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(topk_merge_kernel<1>, mgrid, dim3(256), 0, st, pval, pidx, p.n_splits, m,
-                       topk, top_val, top_idx, merge);
-  } else {
-    hipLaunchKernelGGL(sim_topk_f32_kernel<EBSD_MAX_TOPK>, grid, block, 0, st, Qf, m, Df, n, ld,
-                       p.tiles_per_split, pval, pidx, index_base);
-    EBSD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(topk_merge_kernel<EBSD_MAX_TOPK>, mgrid, dim3(256), 0, st, pval, pidx,
-                       p.n_splits, m, topk, top_val, top_idx, merge);
+  const int ld_bytes = ld * eb;
+  switch (dtype) {
+    case EBSD_DTYPE_F32:
+      return launch_sim<EBSD_DTYPE_F32>(p, q, q_scale, m, dict, d_scale, n, ld_bytes, topk, top_val, top_idx,
+                                        index_base, merge, pval, pidx, st);
+    case EBSD_DTYPE_BF16:
+      return launch_sim<EBSD_DTYPE_BF16>(p, q, q_scale, m, dict, d_scale, n, ld_bytes, topk, top_val, top_idx,
+                                         index_base, merge, pval, pidx, st);
+    case EBSD_DTYPE_F16:
+      return launch_sim<EBSD_DTYPE_F16>(p, q, q_scale, m, dict, d_scale, n, ld_bytes, topk, top_val, top_idx,
+                                        index_base, merge, pval, pidx, st);
+    case EBSD_DTYPE_FP8_E4M3:
+      return launch_sim<EBSD_DTYPE_FP8_E4M3>(p, q, q_scale, m, dict, d_scale, n, ld_bytes, topk, top_val,
+                                             top_idx, index_base, merge, pval, pidx, st);
   }
-  EBSD_LAUNCH_CHECK();
-  return EBSD_OK;
+  return EBSD_ERR_DTYPE;
 }

@@ -99,8 +99,8 @@ def project(master_pattern: torch.Tensor, rays: torch.Tensor, quats: torch.Tenso
     mp = master_pattern.contiguous()
     rays = rays.contiguous()
     quats = quats.contiguous()
-    if mp.dim() != 2 or mp.shape[1] != 2 * mp.shape[0]:
-        raise ValueError(f"master pattern must be (S, 2S), got {tuple(mp.shape)}")
+    if mp.dim() != 2:
+        raise ValueError(f"master pattern must be a 2D side-by-side image, got {tuple(mp.shape)}")
     if rays.dim() != 2 or rays.shape[1] != 3:
         raise ValueError("rays must be (D, 3)")
     if quats.dim() != 2 or quats.shape[1] != 4:

@@ -95,8 +95,6 @@ class MasterPattern(torch.nn.Module):
             mp = self.master_pattern_binned
         else:
             mp = self.master_pattern
-        if mp.shape[1] != 2 * mp.shape[0]:
-            raise ValueError(f"master pattern must be (S, 2S) side by side, got {tuple(mp.shape)}")
         return mp.contiguous().float()
 
     def interpolate(self, coords_3D: torch.Tensor, mode: str = "bilinear", padding_mode: str = "border",

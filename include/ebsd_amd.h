@@ -65,8 +65,9 @@ int ebsd_pad_multiple(int dtype);
  * Rosca-Lambert square, sample the master pattern bilinearly (border padding,
  * align_corners=False), then (flags) zero-mean and/or unit-norm the pattern
  * and store row n of `out` in `out_dtype`.
- *   mp       (mp_rows, 2*mp_rows) f32, northern hemisphere in columns
- *            [0, mp_rows), southern in [mp_rows, 2*mp_rows)
+ *   mp       (mp_rows, mp_cols) f32 side-by-side Lambert image, northern
+ *            hemisphere in the left half, southern in the right half
+ *            (mp_cols = 2 * mp_rows, or the average-pooled shape of one)
  *   rays     (n_pix, 3) f32 sample-frame directions
  *   quats    (n_ori, 4) f32 unit quaternions (w, x, y, z)
  *   out      (n_ori, ld_out) of out_dtype; out_scale (n_ori,) f32 is written

@@ -61,6 +61,8 @@ def band_master_pattern(R, rng, size, n_bands=6):
     yy, xx = np.meshgrid(X, X, indexing="ij")
     sq = torch.tensor(np.stack([xx, yy], -1), dtype=torch.float64)
     nh = R["sphere"].inv_rosca_lambert(sq).numpy()
+    # the square's centre (odd sizes) is 0/0 in the inverse map: the pole
+    nh[np.isnan(nh).any(-1)] = (0.0, 0.0, 1.0)
     normals = unit_vecs(rng, n_bands).astype(np.float64)
     amps = rng.uniform(0.5, 1.0, n_bands)
     sig = np.deg2rad(8.0)

@@ -42,7 +42,7 @@ def test_argument_errors_return_codes_without_launching():
     lib = _lib.load()
     fake = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails validation first
     assert lib.ebsd_project(None, 5, 10, fake, 4, fake, 1, fake, 0, 32, None, 1, None) == _lib.EBSD_ERR_NULL
-    assert lib.ebsd_project(fake, 5, 11, fake, 4, fake, 1, fake, 0, 32, None, 1, None) == _lib.EBSD_ERR_SHAPE
+    assert lib.ebsd_project(fake, 5, 0, fake, 4, fake, 1, fake, 0, 32, None, 1, None) == _lib.EBSD_ERR_SHAPE
     assert lib.ebsd_project(fake, 5, 10, fake, 40, fake, 1, fake, 0, 32, None, 1, None) == _lib.EBSD_ERR_SHAPE
     assert lib.ebsd_project(fake, 5, 10, fake, 4, fake, 1, fake, 0, 33, None, 1, None) == _lib.EBSD_ERR_SHAPE
     assert lib.ebsd_project(fake, 5, 10, fake, 4, fake, 1, fake, 7, 32, None, 1, None) == _lib.EBSD_ERR_DTYPE

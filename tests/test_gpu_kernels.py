@@ -147,7 +147,7 @@ def test_sim_topk_streaming_merge_equals_one_pass():
     v2 = torch.full((200, k), float("-inf"), device=DEV)
     i2 = torch.full((200, k), -1, dtype=torch.int32, device=DEV)
     for s in range(0, 7000, 999):
-        kernels.sim_topk(Q, D.narrow(0, s, min(999, 7000 - s)), k, v2, i2, index_base=s, merge=True)
+        kernels.sim_topk(Q, D.narrow(s, min(999, 7000 - s)), k, v2, i2, index_base=s, merge=True)
     assert torch.equal(i1, i2) and torch.equal(v1, v2)
     # the ChunkedKNN wrapper: chunked data and queries, global indices
     kn = E.ChunkedKNN(7000, 200, 5, DEV, match_dtype=torch.float32)
@@ -188,3 +188,21 @@ def test_rescore_exact_fp32():
     rv, ri = torch.topk(full, 4, dim=1)
     assert torch.allclose(v.cpu().double(), rv, atol=1e-4)
     assert torch.equal(i.cpu().long(), cand.long().gather(1, ri))
+
+
+@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_F16, DTYPE_FP8_E4M3])
+@pytest.mark.parametrize("m,n,d,k", [(300, 5000, 200, 4), (129, 3000, 3600, 16), (1000, 9000, 96, 1)])
+def test_sim_topk_low_precision_exact_on_stored_values(dtype, m, n, d, k):
+    """Scores must equal the float64 dot products of the stored (rounded)
+    values, scaled: products of BF16/FP16/FP8 are exact in FP32, so only the
+    FP32 accumulation rounds."""
+    _say(f"sim_topk dtype={dtype} m={m} n={n} d={d} k={k}")
+    g = torch.Generator().manual_seed(m * 7 + n + d + k + dtype)
+    q = torch.randn(m, d, generator=g)
+    dd = torch.randn(n, d, generator=g)
+    Q = kernels.prep_patterns(q.to(DEV), dtype, zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), dtype, zero_mean=False)
+    val = torch.empty(m, k, device=DEV)
+    idx = torch.empty(m, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, val, idx, merge=False)
+    _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
