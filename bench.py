@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp8"])
+    p.add_argument("--precision", default="fp8", choices=["fp32", "bf16", "fp16", "fp8"])
     p.add_argument("--rescore-k", type=int, default=16)
     p.add_argument("--dict-size", type=int, default=1_000_000)
     p.add_argument("--batch", type=int, default=65536)
@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--agree-sample", type=int, default=4096,
                    help="patterns re-indexed in plain FP32 to measure top-1 agreement")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="first run a 1 GiB device copy (reference kernel for PMC byte counters)")
     return p.parse_args()
 
 
@@ -70,6 +72,12 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     det = a.detector
     D = det * det
 
@@ -106,11 +114,6 @@ def main():
 
     need_f32 = eng.dictionary_f32 is not None
 
-    def step():
-        q = eng.prepare(exp)
-        qf = eng.prepare(exp, dtype=DTYPE_F32) if need_f32 else None
-        return eng.index(q, a.topk, qf)
-
     # kernel-level timing of the similarity call (events on torch's current
     # stream, where the library launches)
     sim_ms = []
@@ -144,15 +147,22 @@ def main():
     value = a.batch * a.steps / wall
     log(f"{a.precision}: {ms_per_step:.1f} ms/step, similarity {sim_avg_ms:.1f} ms, {value:.0f} patterns/s")
 
-    # ---- agreement with plain FP32 DI on a sample ----
+    # ---- agreement with plain FP32 DI on a sample (also times FP32 DI) ----
     agree = None
+    fp32_rate = None
     if a.precision != "fp32" and a.agree_sample > 0:
         ref = E.DictionaryIndexer(mp, rays, dico, precision="fp32")
         ns = min(a.agree_sample, a.batch)
-        rv, ri = ref.index(ref.prepare(exp[:ns]), 1)
+        qs = ref.prepare(exp[:ns])
+        ref.index(qs, 1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rv, ri = ref.index(ref.prepare(exp[:ns]), a.topk)
+        torch.cuda.synchronize()
+        fp32_rate = ns / (time.perf_counter() - t1)
         agree = float((ri[:, 0] == idx[:ns, 0]).float().mean().item())
         del ref
-        log(f"top-1 agreement with FP32 over {ns} patterns: {agree:.5f}")
+        log(f"top-1 agreement with FP32 over {ns} patterns: {agree:.5f}; FP32 DI {fp32_rate:.0f} patterns/s")
 
     flops = 2.0 * a.batch * n_dict * D
     achieved = flops / (sim_avg_ms / 1000) / 1e12
@@ -185,6 +195,10 @@ def main():
             "precision": a.precision, "rescore_k": rk,
         },
         "top1_agreement_vs_fp32": 1.0 if a.precision == "fp32" else agree,
+        "agreement_sample": None if a.precision == "fp32" else min(a.agree_sample, a.batch),
+        "fp32_di_patterns_per_s": round(value, 1) if a.precision == "fp32" else (
+            None if fp32_rate is None else round(fp32_rate, 1)),
+        "speedup_vs_fp32_di": None if fp32_rate is None else round(value / fp32_rate, 2),
         "roofline": {
             "kernel": "sim_topk (similarity GEMM + fused top-k)",
             "bound": "mfma",

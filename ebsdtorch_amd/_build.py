@@ -2,10 +2,7 @@
 
 Each HIP source is compiled to an object with hipcc and the objects are
 linked into one shared library next to this file, so the library ships with
-the source tree. Per-file flags:
-  project.hip  -ffp-contract=off: every multiply and add rounds separately,
-               like the step-by-step torch expressions it restates.
-Packed FP32 instructions are disabled for all device code.
+the source tree. Packed FP32 instructions are disabled for all device code.
 """
 
 from __future__ import annotations
@@ -23,7 +20,7 @@ ARCH = "gfx950"
 
 SOURCES = {
     "abi.hip": [],
-    "project.hip": ["-ffp-contract=off"],
+    "project.hip": [],
     "simtopk.hip": [],
 }
 
