@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--topk", type=int, default=16)
     p.add_argument("--mp-size", type=int, default=501)
     p.add_argument("--detector", type=int, default=60)
-    p.add_argument("--agree-sample", type=int, default=4096,
+    p.add_argument("--agree-sample", type=int, default=16384,
                    help="patterns re-indexed in plain FP32 to measure top-1 agreement")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
     p.add_argument("--calibrate-copy", action="store_true",
