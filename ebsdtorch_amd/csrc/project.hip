@@ -12,9 +12,10 @@
 // every output byte is written exactly once and coalesced. The master pattern
 // (2 x S^2 floats, 2 MB at S = 501) stays resident in each XCD's L2.
 //
-// Numerics: each pixel value agrees with the reference's float32 torch
-// pipeline to within a few float32 ulps of the sampling coordinate (checked
-// against the oracle to 2e-5 absolute on [0, 1] master patterns).
+// Numerics: rotation, Lambert arithmetic and the bilinear blend round step by
+// step like the reference's float32 torch expressions (contraction is off
+// inside them); the atan of the Lambert map is this library's own
+// polynomial (<= 1.44 ulp) evaluated on a reciprocal-based ratio.
 
 #include "common.h"
 
@@ -26,36 +27,22 @@ namespace {
 
 constexpr float kFp8Max = 448.0f;  // largest finite OCP e4m3fn value
 
-// Rotation matrix of unit quaternion (w, x, y, z), active convention
-// v' = q (0, v) q* (quaternions.py:186-213 computes the same map through two
-// quaternion products; orientations.py:911-937 gives this matrix form).
-// Built once per pattern, applied with 9 multiply-adds per pixel.
-struct Rot3 {
-  float m[9];
-};
-
-__device__ __forceinline__ Rot3 quat_to_matrix(float w, float x, float y, float z) {
-  Rot3 r;
-  const float xx = x * x, yy = y * y, zz = z * z;
-  const float xy = x * y, xz = x * z, yz = y * z;
-  const float wx = w * x, wy = w * y, wz = w * z;
-  r.m[0] = 1.0f - 2.0f * (yy + zz);
-  r.m[1] = 2.0f * (xy - wz);
-  r.m[2] = 2.0f * (xz + wy);
-  r.m[3] = 2.0f * (xy + wz);
-  r.m[4] = 1.0f - 2.0f * (xx + zz);
-  r.m[5] = 2.0f * (yz - wx);
-  r.m[6] = 2.0f * (xz - wy);
-  r.m[7] = 2.0f * (yz + wx);
-  r.m[8] = 1.0f - 2.0f * (xx + yy);
-  return r;
-}
-
-__device__ __forceinline__ void rotate_ray(const Rot3& r, float bx, float by, float bz,
+// Rotate p by unit quaternion (w, x, y, z) with the exact operation order of
+// qu_apply (quaternions.py:203-211): q * (1, p) * conj(q), vector part. Every
+// product and sum rounds separately, as in the reference's elementwise torch
+// ops; a rotation-matrix form is 4x cheaper but moves sampling coordinates by
+// ~3e-7, enough to change values on steep master-pattern bands by ~1e-4.
+__device__ __forceinline__ void rotate_ray(float aw, float ax, float ay, float az,
+                                           float bx, float by, float bz,
                                            float& ox, float& oy, float& oz) {
-  ox = fmaf(r.m[0], bx, fmaf(r.m[1], by, r.m[2] * bz));
-  oy = fmaf(r.m[3], bx, fmaf(r.m[4], by, r.m[5] * bz));
-  oz = fmaf(r.m[6], bx, fmaf(r.m[7], by, r.m[8] * bz));
+#pragma clang fp contract(off)
+  const float iw = ((aw - ax * bx) - ay * by) - az * bz;
+  const float ix = ((aw * bx + ax) + ay * bz) - az * by;
+  const float iy = ((aw * by - ax * bz) + ay) + az * bx;
+  const float iz = ((aw * bz + ax * by) - ay * bx) + az;
+  ox = (((-iw) * ax + ix * aw) - iy * az) + iz * ay;
+  oy = (((-iw) * ay + ix * az) + iy * aw) - iz * ax;
+  oz = (((-iw) * az - ix * ay) + iy * ax) + iz * aw;
 }
 
 // atan(t) for |t| <= 1: t * P(t^2), an 11-term least-maximum-error fit of
@@ -82,6 +69,7 @@ __device__ __forceinline__ float atan_unit(float t) {
 // S rows. The reference's atan2(sml * sgn, |big|) always has
 // |sml| <= |big|, so it is atan of a ratio in [-1, 1].
 __device__ __forceinline__ void lambert_sbs(float x, float y, float z, float& gx, float& gy) {
+#pragma clang fp contract(off)
   const float factor = __builtin_amdgcn_sqrtf(fmaxf(2.0f * (1.0f - fabsf(z)), 0.0f));
   const bool cond = fabsf(y) <= fabsf(x);
   const float big = cond ? x : y;
@@ -103,6 +91,7 @@ __device__ __forceinline__ void lambert_sbs(float x, float y, float z, float& gx
 // nw*v_nw + ne*v_ne + sw*v_sw + se*v_se (the order of torch's kernel).
 __device__ __forceinline__ float sample_border(const float* __restrict__ img, int H, int W,
                                                float gx, float gy) {
+#pragma clang fp contract(off)
   float ix = (gx + 1.0f) * (0.5f * (float)W) - 0.5f;
   float iy = (gy + 1.0f) * (0.5f * (float)H) - 0.5f;
   ix = fminf(fmaxf(ix, 0.0f), (float)(W - 1));
@@ -124,7 +113,7 @@ __device__ __forceinline__ float sample_border(const float* __restrict__ img, in
   const float v_ne = __ldg(img + (y0 * W + x1));
   const float v_sw = __ldg(img + (y1 * W + x0));
   const float v_se = __ldg(img + (y1 * W + x1));
-  return fmaf(v_se, n * w, fmaf(v_sw, n * e, fmaf(v_ne, s * w, v_nw * (s * e))));
+  return ((v_nw * (s * e) + v_ne * (s * w)) + v_sw * (n * e)) + v_se * (n * w);
 }
 
 // x / |x| with the squared norm accumulated as fma(z, z, fma(y, y, x*x)),
@@ -226,7 +215,6 @@ __global__ __launch_bounds__(256) void project_kernel(
   }
   for (int n = blockIdx.x; n < n_ori; n += gridDim.x) {
     const float4 q = *reinterpret_cast<const float4*>(quats + 4 * (size_t)n);
-    const Rot3 rot = quat_to_matrix(q.x, q.y, q.z, q.w);
     float vals[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -244,7 +232,7 @@ __global__ __launch_bounds__(256) void project_kernel(
           bz = __ldg(rays + 3 * d + 2);
         }
         float x, y, z;
-        rotate_ray(rot, bx, by, bz, x, y, z);
+        rotate_ray(q.x, q.y, q.z, q.w, bx, by, bz, x, y, z);
         if (renorm) renormalise(x, y, z);
         float gx, gy;
         lambert_sbs(x, y, z, gx, gy);
