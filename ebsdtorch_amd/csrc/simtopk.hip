@@ -12,9 +12,10 @@
 // 128 bytes of K per row (32 FP32, 64 BF16/FP16 or 128 FP8 elements).
 //
 // Work decomposition
-//   grid.x  : query tiles of 128 rows
-//   grid.y  : dictionary splits; split s walks dictionary tiles
-//             [s * tiles_per_split, (s + 1) * tiles_per_split) of 128 rows
+//   grid    : 512 persistent workgroups = 8 XCDs x 64 resident slots; each
+//             XCD works through groups of A query tiles (128 rows each) x B
+//             dictionary parts (A * B = 64), so that both operands of its
+//             concurrent workgroups are shared in its L2 (plan_sim)
 //   block   : 256 threads = 4 waves; wave w owns queries [32 w, 32 w + 32)
 //             of the block and all 128 dictionary rows of the current tile,
 //             i.e. a 128 x 32 score block = 4 MFMA 32x32 tiles (64 acc VGPRs)
@@ -109,20 +110,56 @@ __device__ __forceinline__ void stage_mfma(const unsigned char* As, const unsign
   }
 }
 
+// Work schedule of the persistent similarity kernel (see plan_sim).
+struct SimSched {
+  int A;               // query tiles per XCD block
+  int B;               // dictionary parts per XCD block (A * B = 64 slots)
+  int n_qtiles;
+  int n_qgroups;       // ceil(n_qtiles / A)
+  int R;               // dictionary super-parts when fewer than 8 query groups
+  int tiles_per_part;  // dictionary tiles per part (n_parts = B * R parts)
+};
+
 template <int DT, int K>
 __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
     const unsigned char* __restrict__ Q, const float* __restrict__ q_scale, int M,
     const unsigned char* __restrict__ Dm, const float* __restrict__ d_scale, int N, int ld_bytes,
-    int tiles_per_split, float* __restrict__ pval, int* __restrict__ pidx, int index_base) {
+    SimSched sc, float* __restrict__ pval, int* __restrict__ pidx, int index_base) {
   __shared__ __attribute__((aligned(16))) unsigned char smem[LDS_BYTES];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int q0 = blockIdx.x * TILE;
-  const int split = blockIdx.y;
+  // Workgroups are dispatched round-robin over the 8 XCDs, so blockIdx % 8 is
+  // this workgroup's XCD and blockIdx / 8 its slot among that XCD's 64.
+  // Slot (a, b) takes query tile a of the XCD's current group of A tiles and
+  // dictionary part b: the B workgroups of a query tile share its stages in
+  // the XCD's L2, and the A workgroups of a part stream the same dictionary
+  // tiles in the same order.
+  const int xcd = blockIdx.x & 7;
+  const int slot = blockIdx.x >> 3;
+  const int a = slot / sc.B;
+  const int b = slot - a * sc.B;
+  int g_first, g_step, super;
+  if (sc.R == 1) {
+    g_first = xcd;
+    g_step = 8;
+    super = 0;
+  } else {
+    g_first = xcd % sc.n_qgroups;
+    g_step = sc.n_qgroups;
+    super = xcd / sc.n_qgroups;
+    if (super >= sc.R) return;
+  }
+  const int part = super * sc.B + b;
   const int n_tiles = (N + TILE - 1) / TILE;
-  const int t_begin = split * tiles_per_split;
-  const int t_end = min(t_begin + tiles_per_split, n_tiles);
+  const int t_begin = part * sc.tiles_per_part;
+  const int t_end = min(t_begin + sc.tiles_per_part, n_tiles);
+
+  for (int g = g_first; g < sc.n_qgroups; g += g_step) {
+  const int qt = g * sc.A + a;
+  if (qt >= sc.n_qtiles) continue;  // uniform over the workgroup
+  const int q0 = qt * TILE;
+  const int split = part;
 
   // Global->LDS mapping: thread copies 16 bytes at byte lc of rows lr + 32 i
   // (i < 4) of both the dictionary tile and the query tile.
@@ -245,6 +282,9 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
       h1 += take1 ? 1 : 0;
     }
   }
+  // the LDS lists are read before the next work unit reuses the stages
+  __syncthreads();
+  }
 }
 
 // Merge n_splits sorted partial lists (layout [split][K][M]) and, if `merge`,
@@ -320,29 +360,37 @@ __global__ __launch_bounds__(256) void rescore_kernel(
 }
 
 struct SimPlan {
-  int k_list;           // register list length (1 or 16)
-  int n_qtiles;
-  int n_splits;
-  int tiles_per_split;
+  int k_list;  // register list length (1 or 16)
+  SimSched sc;
+  int n_parts;  // partial lists per query (= B * R)
+  int grid;
 };
+
+// 8 XCDs x 32 CUs x 2 resident workgroups on MI355X: one persistent
+// workgroup per resident slot.
+constexpr int kXcds = 8;
+constexpr int kSlotsPerXcd = 64;
 
 SimPlan plan_sim(int m, int n, int topk) {
   SimPlan p;
   p.k_list = topk <= 1 ? 1 : EBSD_MAX_TOPK;
-  p.n_qtiles = (m + TILE - 1) / TILE;
+  SimSched& s = p.sc;
+  s.n_qtiles = (m + TILE - 1) / TILE;
   const int n_tiles = (n + TILE - 1) / TILE;
-  // Aim for >= 2048 workgroups (256 CUs x 2 resident x 4 rounds) so the
-  // dictionary walk is balanced, but never split below one tile.
-  int splits = (2048 + p.n_qtiles - 1) / p.n_qtiles;
-  if (splits > n_tiles) splits = n_tiles;
-  if (splits < 1) splits = 1;
-  p.tiles_per_split = (n_tiles + splits - 1) / splits;
-  p.n_splits = (n_tiles + p.tiles_per_split - 1) / p.tiles_per_split;
+  // Largest A <= 8 that still gives every XCD a query group.
+  s.A = 8;
+  while (s.A > 1 && (s.n_qtiles + s.A - 1) / s.A < kXcds) s.A >>= 1;
+  s.B = kSlotsPerXcd / s.A;
+  s.n_qgroups = (s.n_qtiles + s.A - 1) / s.A;
+  s.R = s.n_qgroups >= kXcds ? 1 : kXcds / s.n_qgroups;
+  p.n_parts = s.B * s.R;
+  s.tiles_per_part = (n_tiles + p.n_parts - 1) / p.n_parts;
+  p.grid = kXcds * kSlotsPerXcd;
   return p;
 }
 
 size_t workspace_bytes(const SimPlan& p, int m) {
-  return (size_t)p.n_splits * p.k_list * (size_t)m * (sizeof(float) + sizeof(int));
+  return (size_t)p.n_parts * p.k_list * (size_t)m * (sizeof(float) + sizeof(int));
 }
 
 }  // namespace
@@ -385,21 +433,21 @@ template <int DT>
 static int launch_sim(const SimPlan& p, const void* q, const float* q_scale, int m, const void* dict,
                       const float* d_scale, int n, int ld_bytes, int topk, float* top_val, int* top_idx,
                       int index_base, int merge, float* pval, int* pidx, hipStream_t st) {
-  const dim3 grid(p.n_qtiles, p.n_splits), block(256);
+  const dim3 grid(p.grid), block(256);
   const dim3 mgrid((m + 255) / 256);
   const unsigned char* Qb = static_cast<const unsigned char*>(q);
   const unsigned char* Db = static_cast<const unsigned char*>(dict);
   if (p.k_list == 1) {
     hipLaunchKernelGGL((sim_topk_kernel<DT, 1>), grid, block, 0, st, Qb, q_scale, m, Db, d_scale, n, ld_bytes,
-                       p.tiles_per_split, pval, pidx, index_base);
+                       p.sc, pval, pidx, index_base);
     EBSD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(topk_merge_kernel<1>, mgrid, dim3(256), 0, st, pval, pidx, p.n_splits, m, topk, top_val,
+    hipLaunchKernelGGL(topk_merge_kernel<1>, mgrid, dim3(256), 0, st, pval, pidx, p.n_parts, m, topk, top_val,
                        top_idx, merge);
   } else {
     hipLaunchKernelGGL((sim_topk_kernel<DT, EBSD_MAX_TOPK>), grid, block, 0, st, Qb, q_scale, m, Db, d_scale, n,
-                       ld_bytes, p.tiles_per_split, pval, pidx, index_base);
+                       ld_bytes, p.sc, pval, pidx, index_base);
     EBSD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(topk_merge_kernel<EBSD_MAX_TOPK>, mgrid, dim3(256), 0, st, pval, pidx, p.n_splits, m,
+    hipLaunchKernelGGL(topk_merge_kernel<EBSD_MAX_TOPK>, mgrid, dim3(256), 0, st, pval, pidx, p.n_parts, m,
                        topk, top_val, top_idx, merge);
   }
   EBSD_LAUNCH_CHECK();
@@ -424,7 +472,7 @@ extern "C" int ebsd_sim_topk(const void* q, const float* q_scale, int m, const v
   const SimPlan p = plan_sim(m, n, topk);
   if (!workspace || ws_bytes < workspace_bytes(p, m)) return EBSD_ERR_WORKSPACE;
   float* pval = static_cast<float*>(workspace);
-  int* pidx = reinterpret_cast<int*>(pval + (size_t)p.n_splits * p.k_list * m);
+  int* pidx = reinterpret_cast<int*>(pval + (size_t)p.n_parts * p.k_list * m);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int ld_bytes = ld * eb;
   switch (dtype) {
