@@ -7,7 +7,9 @@ drivers for the dictionary-indexing path.
 
 from .experiment_pats import ExperimentPatterns
 from .geometry import EBSDGeometry, bruker_geometry_to_SE3
-from .indexing import DictionaryIndexer, dictionary_index_orientations
+from .indexing import (DictionaryIndexer, PCADictionaryIndexer, dictionary_index_orientations,
+                       pca_dictionary_index_orientations)
+from .pca_covmat import OnlineCovMatrix
 from .knn import ChunkedKNN
 from .master_pattern import MasterPattern, point_group_to_laue, space_group_to_laue
 from .orientations import (
@@ -24,7 +26,7 @@ from .orientations import (
 )
 
 __all__ = [
-    "ChunkedKNN", "DictionaryIndexer", "EBSDGeometry", "ExperimentPatterns", "MasterPattern",
+    "ChunkedKNN", "DictionaryIndexer", "OnlineCovMatrix", "PCADictionaryIndexer", "pca_dictionary_index_orientations", "EBSDGeometry", "ExperimentPatterns", "MasterPattern",
     "bruker_geometry_to_SE3", "dictionary_index_orientations", "disori_angle_laue", "get_laue_mult",
     "laue_elements", "ori_in_fz_laue", "ori_to_fz_laue", "point_group_to_laue", "qu_apply",
     "sample_ori_fz_laue", "sample_ori_fz_laue_angle", "so3_cubochoric_grid", "so3_uniform_quat",

@@ -22,6 +22,7 @@ SOURCES = {
     "abi.hip": [],
     "project.hip": [],
     "simtopk.hip": [],
+    "pca.hip": [],
 }
 
 COMMON_FLAGS = [

@@ -66,6 +66,13 @@ _SIGNATURES = {
     "ebsd_rescore": (
         _c_int,
         [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p]),
+    "ebsd_gemm_nt": (
+        _c_int, [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, ctypes.c_float,
+                 ctypes.c_float, _c_void_p]),
+    "ebsd_cov_update_workspace": (_c_int, [_c_int, _c_int, ctypes.POINTER(_c_size_t)]),
+    "ebsd_cov_update": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_int, ctypes.c_longlong, _c_void_p,
+                 _c_size_t, _c_void_p]),
     "ebsd_sim_topk_workspace": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_c_size_t)]),
     "ebsd_sim_topk": (
         _c_int,

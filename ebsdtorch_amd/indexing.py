@@ -27,12 +27,6 @@ from .knn import ChunkedKNN
 from .master_pattern import MasterPattern
 from .orientations import sample_ori_fz_laue_angle
 
-# Dictionary rows projected per launch when the engine streams a dictionary
-# that it does not keep resident.
-_STREAM_ROWS = 1 << 17
-# Keep a projected dictionary resident up to this many bytes.
-_RESIDENT_BYTES = 48 << 30
-
 
 def _gpu_device(t: torch.Tensor) -> torch.device:
     if t.device.type == "cuda":
@@ -42,7 +36,30 @@ def _gpu_device(t: torch.Tensor) -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
-class DictionaryIndexer:
+class _ResidentSearch:
+    """Top-k search of prepared queries against a resident pattern matrix
+    (self.dictionary in the search dtype, optional FP32 copy for rescoring)."""
+
+    dictionary: kernels.PatternMatrix
+    dictionary_f32: Optional[kernels.PatternMatrix]
+    rescore_k: int
+    device: torch.device
+
+    def index(self, queries: kernels.PatternMatrix, topk: int, queries_f32: Optional[kernels.PatternMatrix] = None):
+        """Top-k (values (N, k) f32, indices (N, k) int32) of prepared queries."""
+        m = queries.rows
+        k_search = max(topk, self.rescore_k) if self.dictionary_f32 is not None else topk
+        val = torch.empty((m, k_search), dtype=torch.float32, device=self.device)
+        idx = torch.empty((m, k_search), dtype=torch.int32, device=self.device)
+        kernels.sim_topk(queries, self.dictionary, k_search, val, idx, index_base=0, merge=False)
+        if self.dictionary_f32 is not None:
+            if queries_f32 is None:
+                raise ValueError("FP32 rescoring needs the FP32 queries")
+            val, idx = kernels.rescore(queries_f32, self.dictionary_f32, idx, topk)
+        return val, idx
+
+
+class DictionaryIndexer(_ResidentSearch):
     """A projected orientation dictionary resident on one GPU.
 
     master_pattern (S, 2S) f32, rays (D, 3) f32 unit sample-frame
@@ -88,18 +105,93 @@ class DictionaryIndexer:
         return kernels.prep_patterns(patterns, self.dtype if dtype is None else dtype, zero_mean=zero_mean,
                                      pix_idx=pix_idx)
 
-    def index(self, queries: kernels.PatternMatrix, topk: int, queries_f32: Optional[kernels.PatternMatrix] = None):
-        """Top-k (values (N, k) f32, indices (N, k) int32) of prepared queries."""
-        m = queries.rows
-        k_search = max(topk, self.rescore_k) if self.dictionary_f32 is not None else topk
-        val = torch.empty((m, k_search), dtype=torch.float32, device=self.device)
-        idx = torch.empty((m, k_search), dtype=torch.int32, device=self.device)
-        kernels.sim_topk(queries, self.dictionary, k_search, val, idx, index_base=0, merge=False)
-        if self.dictionary_f32 is not None:
-            if queries_f32 is None:
-                raise ValueError("FP32 rescoring needs the FP32 queries")
-            val, idx = kernels.rescore(queries_f32, self.dictionary_f32, idx, topk)
-        return val, idx
+
+
+class PCADictionaryIndexer(_ResidentSearch):
+    """PCA dictionary indexing engine (indexing.py:214-476).
+
+    The covariance of the zero-mean learning-dictionary patterns is
+    accumulated on the matrix cores (OnlineCovMatrix), its eigenvectors are
+    solved in FP64, the chosen components (columns, largest eigenvalue last)
+    form P (D x n_components), and the index dictionary is projected and
+    stored as its PCA coefficients (N x n_components) in HBM. Queries are
+    zero-meaned, projected onto P and searched like plain DI.
+
+    coords are the un-normalised sample-frame pixel positions: the rotated
+    directions are normalised inside the projection kernel, as
+    interpolate(normalize_coords=True) does (indexing.py:323-335).
+    component_slice "top" keeps the n largest components (the documented
+    intent); "reference" reproduces indexing.py:355's slice [-n : -n + 300].
+    """
+
+    def __init__(self, master_pattern: torch.Tensor, coords: torch.Tensor, learn_orientations: torch.Tensor,
+                 n_components: int, index_orientations: Optional[torch.Tensor] = None, precision: str = "fp32",
+                 rescore_k: int = 0, component_slice: str = "top", chunk: int = 1 << 16):
+        from .pca_covmat import OnlineCovMatrix
+
+        if precision not in kernels.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(kernels.PRECISIONS)}")
+        self.device = learn_orientations.device
+        self.mp = master_pattern.to(self.device, torch.float32).contiguous()
+        self.coords = coords.to(self.device, torch.float32).contiguous()
+        self.precision = precision
+        self.dtype = kernels.PRECISIONS[precision]
+        self.rescore_k = int(rescore_k)
+        d = self.coords.shape[0]
+        self.n_pix = d
+        learn = learn_orientations.to(self.device, torch.float32).contiguous()
+        self.covariance = OnlineCovMatrix(d, device=self.device)
+        for s in range(0, learn.shape[0], chunk):
+            self.covariance(self._project(learn[s:s + chunk]))
+        self.eigenvalues, evecs = self.covariance.get_eigensystem(torch.float64)
+        if component_slice == "top":
+            comps = evecs[:, -n_components:]
+        elif component_slice == "reference":
+            comps = evecs[:, -n_components:(-n_components + 300)]
+        else:
+            raise ValueError("component_slice must be 'top' or 'reference'")
+        if comps.shape[1] == 0:
+            raise ValueError("the component slice is empty")
+        self.components = comps  # (D, n) float64
+        n = comps.shape[1]
+        self.n_components = n
+        pt = kernels.alloc_patterns(kernels.padded_width(n, DTYPE_F32), d, DTYPE_F32, self.device)
+        pt.data.zero_()
+        pt.data[:n, :d] = comps.T.to(torch.float32)
+        self.components_t = pt
+        index = learn if index_orientations is None else index_orientations.to(self.device, torch.float32)
+        self.orientations = index.contiguous()
+        n_pad = kernels.padded_width(n, DTYPE_F32)
+        f32 = torch.empty((index.shape[0], n_pad), dtype=torch.float32, device=self.device)
+        for s in range(0, index.shape[0], chunk):
+            kernels.gemm_nt(self._project(index[s:s + chunk]), pt, out=f32[s:s + chunk])
+        dict_f32 = kernels.PatternMatrix(f32, None, DTYPE_F32, n)
+        if self.dtype == DTYPE_F32:
+            self.dictionary, self.dictionary_f32 = dict_f32, None
+        else:
+            self.dictionary = kernels.prep_patterns(f32[:, :n], self.dtype, zero_mean=False)
+            self.dictionary_f32 = dict_f32 if self.rescore_k > 0 else None
+
+    def _project(self, quats: torch.Tensor) -> kernels.PatternMatrix:
+        return kernels.project(self.mp, self.coords, quats, DTYPE_F32, zero_mean=True, renorm_dirs=True)
+
+    @property
+    def size(self) -> int:
+        return self.orientations.shape[0]
+
+    def prepare(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None,
+                zero_mean: bool = True, dtype: Optional[int] = None) -> kernels.PatternMatrix:
+        """(N, P) patterns -> PCA coefficients (N, n_components) in the search
+        dtype (or `dtype`)."""
+        q = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, pix_idx=pix_idx)
+        n_pad = self.components_t.rows
+        out = torch.empty((q.rows, n_pad), dtype=torch.float32, device=self.device)
+        kernels.gemm_nt(q, self.components_t, out=out)
+        pm = kernels.PatternMatrix(out, None, DTYPE_F32, self.n_components)
+        dt = self.dtype if dtype is None else dtype
+        if dt == DTYPE_F32:
+            return pm
+        return kernels.prep_patterns(out[:, :self.n_components], dt, zero_mean=False)
 
 
 def dictionary_index_orientations(
@@ -187,4 +279,94 @@ def dictionary_index_orientations(
         if match_dtype in (torch.float16, torch.bfloat16) and precision in ("fp16", "bf16") and rescore_k == 0:
             val = val.to(match_dtype)
         experiment_patterns.set_raw_indexing_results(ori[idx.clamp_min(0)], val, phase_id=i, indices=idx)
+    experiment_patterns.combine_indexing_results(higher_is_better=True)
+
+
+def pca_dictionary_index_orientations(
+    master_patterns: Union[MasterPattern, List[MasterPattern]],
+    geometry: EBSDGeometry,
+    experiment_patterns: ExperimentPatterns,
+    n_pca_components: int,
+    signal_mask: Optional[torch.Tensor] = None,
+    subtract_exp_pat_mean: bool = True,
+    experiment_chunk_size: int = 256,
+    dictionary_resolution_learn_deg: float = 0.5,
+    dictionary_resolution_index_deg: float = 0.5,
+    dictionary_chunk_size: int = 256,
+    virtual_binning: int = 1,
+    top_k_matches: int = 1,
+    distance_metric: str = "angular",
+    match_dtype: torch.dtype = torch.float16,
+    quantized_via_ao: bool = True,
+    average_pattern_center: bool = True,
+    *,
+    learn_orientations: Optional[Sequence[torch.Tensor]] = None,
+    index_orientations: Optional[Sequence[torch.Tensor]] = None,
+    precision: Optional[str] = None,
+    rescore_k: int = 0,
+    component_slice: str = "top",
+    generator: Optional[torch.Generator] = None,
+) -> None:
+    """PCA dictionary indexing (indexing.py:214-476): learn the covariance
+    of a learning dictionary, keep n_pca_components principal components,
+    index in PCA space; results are stored in `experiment_patterns` as by
+    dictionary_index_orientations.
+
+    Extensions (keyword-only): learn_orientations / index_orientations
+    (per-phase dictionaries instead of sampling), precision ("fp32" |
+    "bf16" | "fp16" | "fp8", default from match_dtype), rescore_k (FP32 re-scoring in
+    PCA space), component_slice ("top" = the n largest components, the
+    documented intent; "reference" = the literal slice of indexing.py:355).
+    """
+    if isinstance(master_patterns, MasterPattern):
+        master_patterns = [master_patterns]
+    if distance_metric != "angular":
+        raise NotImplementedError(f"'{distance_metric}' not supported.")
+    if not average_pattern_center:
+        raise NotImplementedError("per-scan-point pattern centres are not supported")
+    if precision is None:
+        if match_dtype not in kernels.MATCH_DTYPES:
+            raise NotImplementedError(f"match_dtype {match_dtype} is not supported")
+        precision = {torch.float32: "fp32", torch.bfloat16: "bf16", torch.float16: "fp16"}[match_dtype]
+    n_exp = experiment_patterns.n_patterns
+    for i, mp in enumerate(master_patterns):
+        device = _gpu_device(mp.master_pattern)
+        if learn_orientations is not None:
+            learn = learn_orientations[i].to(device, torch.float32)
+        else:
+            learn = sample_ori_fz_laue_angle(mp.laue_group, dictionary_resolution_learn_deg, device, permute=True,
+                                             generator=generator)
+        if index_orientations is not None:
+            index = index_orientations[i].to(device, torch.float32)
+        elif dictionary_resolution_learn_deg != dictionary_resolution_index_deg:
+            index = sample_ori_fz_laue_angle(mp.laue_group, dictionary_resolution_index_deg, device, permute=True,
+                                             generator=generator)
+        else:
+            index = learn
+        coords = geometry.get_coords_sample_frame((virtual_binning, virtual_binning))
+        pix_idx = None
+        if signal_mask is not None:
+            flat = signal_mask.flatten().to(coords.device)
+            coords = coords[flat]
+            pix_idx = torch.nonzero(flat.to(device), as_tuple=False).flatten().to(torch.int32)
+        engine = PCADictionaryIndexer(mp.pattern_for_binning(virtual_binning), coords.to(device), learn,
+                                      n_pca_components, index_orientations=index, precision=precision,
+                                      rescore_k=rescore_k, component_slice=component_slice)
+        exp = experiment_patterns.get_patterns(torch.arange(n_exp, device=experiment_patterns.patterns.device),
+                                               binning=virtual_binning)
+        exp = exp.reshape(n_exp, -1).to(device, torch.float32)
+        zero_mean = subtract_exp_pat_mean if experiment_chunk_size > n_exp else True
+        vals, idxs = [], []
+        step = max(1, min(n_exp, (4 << 30) // (4 * max(1, exp.shape[1]))))
+        for s in range(0, n_exp, step):
+            q = engine.prepare(exp[s:s + step], pix_idx, zero_mean)
+            qf = engine.prepare(exp[s:s + step], pix_idx, zero_mean, DTYPE_F32) \
+                if engine.dictionary_f32 is not None else None
+            v, ix = engine.index(q, top_k_matches, qf)
+            vals.append(v)
+            idxs.append(ix)
+        val = torch.cat(vals, 0)
+        idx = torch.cat(idxs, 0).to(torch.int64)
+        experiment_patterns.set_raw_indexing_results(engine.orientations[idx.clamp_min(0)], val, phase_id=i,
+                                                     indices=idx)
     experiment_patterns.combine_indexing_results(higher_is_better=True)

@@ -209,3 +209,43 @@ def rescore(queries_f32: PatternMatrix, dictionary_f32: PatternMatrix, candidate
                                   cand.data_ptr(), c, topk, val.data_ptr(), idx.data_ptr(), _lib.stream_ptr(dev))
     check(st, "ebsd_rescore")
     return val, idx
+
+
+def gemm_nt(a: PatternMatrix, b: PatternMatrix, out: Optional[torch.Tensor] = None, alpha: float = 1.0,
+            beta: float = 0.0) -> torch.Tensor:
+    """C = alpha * A B^T + beta * C on the matrix cores (ebsd_gemm_nt):
+    A (m, ld), B (n, ld) pattern matrices of one dtype -> C (m, n) float32
+    (or written into `out`, whose row stride may exceed n)."""
+    if a.dtype != b.dtype or a.ld != b.ld:
+        raise ValueError("both operands need the same dtype and leading dimension")
+    m, n = a.rows, b.rows
+    dev = a.data.device
+    if out is None:
+        out = torch.empty((m, n), dtype=torch.float32, device=dev)
+    if out.dtype != torch.float32 or out.shape[0] != m or out.shape[1] < n or out.stride(1) != 1:
+        raise ValueError("output must be float32 (m, >= n) with unit column stride")
+    st = _lib.load().ebsd_gemm_nt(a.data.data_ptr(), m, b.data.data_ptr(), n, a.ld, a.dtype, out.data_ptr(),
+                                  out.stride(0), alpha, beta, _lib.stream_ptr(dev))
+    check(st, "ebsd_gemm_nt")
+    return out
+
+
+def cov_update(x: PatternMatrix, mean: torch.Tensor, cov: torch.Tensor, n_seen: int) -> None:
+    """Merge the rows of `x` (first n_pix columns, float32) into the running
+    mean (d,) and covariance aggregate (d, d) in place (ebsd_cov_update);
+    only block tiles on or above the diagonal of `cov` are maintained."""
+    if x.dtype != DTYPE_F32:
+        raise ValueError("covariance batches are float32")
+    d = x.n_pix
+    if mean.shape != (d,) or cov.shape[0] != d or cov.shape[1] < d:
+        raise ValueError("mean / covariance shapes do not match the batch")
+    if not (mean.is_contiguous() and cov.stride(1) == 1):
+        raise ValueError("mean and covariance must be contiguous")
+    lib = _lib.load()
+    nbytes = ctypes.c_size_t(0)
+    check(lib.ebsd_cov_update_workspace(x.rows, d, ctypes.byref(nbytes)), "ebsd_cov_update_workspace")
+    dev = x.data.device
+    ws = _workspace(dev, nbytes.value)
+    st = lib.ebsd_cov_update(x.data.data_ptr(), x.rows, d, x.ld, mean.data_ptr(), cov.data_ptr(), cov.stride(0),
+                             int(n_seen), ws.data_ptr(), ws.numel(), _lib.stream_ptr(dev))
+    check(st, "ebsd_cov_update")

@@ -143,6 +143,34 @@ int ebsd_sim_topk(const void* q, const float* q_scale, int m, const void* dict,
 int ebsd_rescore(const float* q, int m, const float* dict, int n, int ld, const int* cand,
                  int n_cand, int topk, float* out_val, int* out_idx, void* stream);
 
+/*
+ * Dense product C = alpha * A * B^T + beta * C (FP32 accumulation).
+ *   a (m, ld), b (n, ld) of dtype F32 / BF16 / F16, row-major, ld a multiple
+ *   of ebsd_pad_multiple(dtype) with zero padding; c (m, ldc) f32.
+ * Replaces: ebsd/indexing.py:401, :421, :452 (torch.matmul of patterns with
+ * the PCA component matrix; b holds the components as rows).
+ */
+int ebsd_gemm_nt(const void* a, int m, const void* b, int n, int ld, int dtype,
+                 float* c, int ldc, float alpha, float beta, void* stream);
+
+/* Bytes of scratch that ebsd_cov_update needs for a batch of n rows of d. */
+int ebsd_cov_update_workspace(int n, int d, size_t* bytes);
+
+/*
+ * Streaming covariance update (Welford / Chan batched form): merges the
+ * batch x (n, ld_x) f32 (first d columns used) into the running mean (d,)
+ * and aggregate cov (d, ld_cov) f32, given n_seen rows merged before.
+ *   cov += (x - mean_old)^T (x - mean_new); mean <- mean_new
+ * Only the block tiles (128 x 128) on or above the diagonal of cov are
+ * updated; the lower triangle is the transpose of the upper one.
+ * workspace: ebsd_cov_update_workspace bytes, 256-byte aligned.
+ * Replaces: utils/pca_covmat.py:53-91 (update_covmat) and :120-131
+ * (OnlineCovMatrix.forward).
+ */
+int ebsd_cov_update(const float* x, int n, int d, int ld_x, float* mean,
+                    float* cov, int ld_cov, long long n_seen, void* workspace,
+                    size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
