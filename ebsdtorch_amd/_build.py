@@ -23,6 +23,8 @@ SOURCES = {
     "project.hip": [],
     "simtopk.hip": [],
     "pca.hip": [],
+    "noise.hip": [],
+    "nn.hip": [],
 }
 
 COMMON_FLAGS = [

@@ -56,6 +56,14 @@ _SIGNATURES = {
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p,
          _c_int, _c_int, _c_void_p, _c_int, _c_void_p],
     ),
+    "ebsd_project_shifted": (
+        _c_int,
+        [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_void_p,
+         _c_int, _c_int, _c_void_p, _c_int, _c_void_p],
+    ),
+    "ebsd_poisson_gaussian_noise": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, ctypes.c_float,
+                 ctypes.c_ulonglong, ctypes.c_ulonglong, _c_void_p]),
     "ebsd_sample_dirs": (
         _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, ctypes.c_longlong, _c_void_p, _c_int, _c_void_p]),
     "ebsd_prep_patterns": (
@@ -69,10 +77,26 @@ _SIGNATURES = {
     "ebsd_gemm_nt": (
         _c_int, [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, ctypes.c_float,
                  ctypes.c_float, _c_void_p]),
+    "ebsd_gemm_nt_splitk_workspace": (_c_int, [_c_int, _c_int, _c_int, ctypes.POINTER(_c_size_t)]),
+    "ebsd_gemm_nt_splitk": (
+        _c_int, [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, ctypes.c_float,
+                 ctypes.c_float, _c_int, _c_void_p, _c_size_t, _c_void_p]),
     "ebsd_cov_update_workspace": (_c_int, [_c_int, _c_int, ctypes.POINTER(_c_size_t)]),
     "ebsd_cov_update": (
         _c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_int, ctypes.c_longlong, _c_void_p,
                  _c_size_t, _c_void_p]),
+    "ebsd_nn_im2col": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_int, _c_void_p]),
+    "ebsd_nn_col2im": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p]),
+    "ebsd_nn_bias_act": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p]),
+    "ebsd_nn_relu_bwd": (_c_int, [_c_void_p, _c_void_p, ctypes.c_longlong, _c_void_p]),
+    "ebsd_nn_colsum_workspace": (_c_int, [_c_int, _c_int, ctypes.POINTER(_c_size_t)]),
+    "ebsd_nn_colsum": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_size_t, _c_void_p]),
+    "ebsd_nn_transpose": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_int, _c_void_p]),
+    "ebsd_nn_geodesic_loss": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_void_p]),
+    "ebsd_nn_adam": (
+        _c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, ctypes.c_longlong, ctypes.c_float, ctypes.c_float,
+                 ctypes.c_float, ctypes.c_float, _c_int, ctypes.c_float, _c_void_p, _c_void_p]),
     "ebsd_sim_topk_workspace": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_c_size_t)]),
     "ebsd_sim_topk": (
         _c_int,

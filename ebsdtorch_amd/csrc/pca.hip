@@ -81,15 +81,25 @@ __global__ __launch_bounds__(256) void center_transpose_kernel(const float* __re
 // C[i][j] = alpha * (A B^T)[i][j] + beta * C[i][j] + w * u[i] * v[j]
 // for the 128 x 128 block (blockIdx.x, blockIdx.y). upper_only skips blocks
 // strictly below the block diagonal.
+// Split K: blockIdx.z = s covers stages [s * split_stages, (s + 1) *
+// split_stages) of the K loop and writes its own output slice
+// C + s * split_stride (alpha, beta, u, v apply per slice; the split-K path
+// writes partial sums with beta = 0 and reduces them afterwards).
 template <int DT>
 __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(const unsigned char* __restrict__ A, int M,
                                                          const unsigned char* __restrict__ B, int N, int ld_bytes,
                                                          float* __restrict__ C, int ldc, float alpha, float beta,
                                                          const float* __restrict__ u, const float* __restrict__ v,
-                                                         float w, int upper_only) {
+                                                         float w, int upper_only, int split_stages,
+                                                         int64_t split_stride) {
   __shared__ __attribute__((aligned(16))) unsigned char smem[LDS_BYTES];
   const int bi_tile = blockIdx.x, bj_tile = blockIdx.y;
   if (upper_only && bi_tile > bj_tile) return;  // uniform over the workgroup
+  const int total_stages = ld_bytes / STAGE_BYTES;
+  const int s0 = blockIdx.z * split_stages;
+  const int ns = min(split_stages, total_stages - s0);
+  if (ns <= 0) return;
+  C += blockIdx.z * split_stride;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -100,14 +110,14 @@ __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(const unsigned char* __
   const unsigned char* brow[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    arow[i] = A + (size_t)min(i0 + lr + 32 * i, M - 1) * ld_bytes + lc;
-    brow[i] = B + (size_t)min(j0 + lr + 32 * i, N - 1) * ld_bytes + lc;
+    arow[i] = A + (size_t)min(i0 + lr + 32 * i, M - 1) * ld_bytes + lc + (size_t)s0 * STAGE_BYTES;
+    brow[i] = B + (size_t)min(j0 + lr + 32 * i, N - 1) * ld_bytes + lc + (size_t)s0 * STAGE_BYTES;
   }
   const int a_off = (lane & 31) * LDS_ROW + (lane >> 5) * 64;
   const int b_off = (wave * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 64;
   const int st_off = lr * LDS_ROW + lc;
   floatx16 acc[4];
-  tile_product<DT>(arow, brow, ld_bytes / STAGE_BYTES, smem, a_off, b_off, st_off, acc);
+  tile_product<DT>(arow, brow, ns, smem, a_off, b_off, st_off, acc);
   const int col = j0 + wave * 32 + (lane & 31);
   if (col >= N) return;
   const float vj = (v && w != 0.0f) ? w * v[col] : 0.0f;
@@ -136,24 +146,39 @@ int elem_bytes(int dtype) {
   return 0;
 }
 
+// out[i][j] = alpha * sum_s part[s][i][j] + beta * out[i][j]
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restrict__ part, int splits, int m,
+                                                            int n, float* __restrict__ out, int ldc, float alpha,
+                                                            float beta) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)m * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += part[(int64_t)k * m * n + idx];
+  float* o = out + (int64_t)i * ldc + j;
+  *o = beta != 0.0f ? alpha * s + beta * *o : alpha * s;
+}
+
 int launch_gemm(const void* a, int m, const void* b, int n, int ld, int dtype, float* c, int ldc, float alpha,
-                float beta, const float* u, const float* v, float w, int upper_only, hipStream_t st) {
+                float beta, const float* u, const float* v, float w, int upper_only, hipStream_t st,
+                int splits = 1, int split_stages = 0, int64_t split_stride = 0) {
   const int eb = elem_bytes(dtype);
-  const dim3 grid((m + TILE - 1) / TILE, (n + TILE - 1) / TILE), block(256);
+  if (splits <= 1) split_stages = ld * eb / STAGE_BYTES;
+  const dim3 grid((m + TILE - 1) / TILE, (n + TILE - 1) / TILE, splits > 1 ? splits : 1), block(256);
   const unsigned char* A = static_cast<const unsigned char*>(a);
   const unsigned char* B = static_cast<const unsigned char*>(b);
   switch (dtype) {
     case EBSD_DTYPE_F32:
       hipLaunchKernelGGL(gemm_nt_kernel<EBSD_DTYPE_F32>, grid, block, 0, st, A, m, B, n, ld * eb, c, ldc, alpha,
-                         beta, u, v, w, upper_only);
+                         beta, u, v, w, upper_only, split_stages, split_stride);
       break;
     case EBSD_DTYPE_BF16:
       hipLaunchKernelGGL(gemm_nt_kernel<EBSD_DTYPE_BF16>, grid, block, 0, st, A, m, B, n, ld * eb, c, ldc, alpha,
-                         beta, u, v, w, upper_only);
+                         beta, u, v, w, upper_only, split_stages, split_stride);
       break;
     case EBSD_DTYPE_F16:
       hipLaunchKernelGGL(gemm_nt_kernel<EBSD_DTYPE_F16>, grid, block, 0, st, A, m, B, n, ld * eb, c, ldc, alpha,
-                         beta, u, v, w, upper_only);
+                         beta, u, v, w, upper_only, split_stages, split_stride);
       break;
     default:
       return EBSD_ERR_DTYPE;
@@ -193,6 +218,39 @@ extern "C" int ebsd_gemm_nt(const void* a, int m, const void* b, int n, int ld, 
   if (m == 0 || n == 0) return EBSD_OK;
   return launch_gemm(a, m, b, n, ld, dtype, c, ldc, alpha, beta, nullptr, nullptr, 0.0f, 0,
                      static_cast<hipStream_t>(stream));
+}
+
+extern "C" int ebsd_gemm_nt_splitk_workspace(int m, int n, int splits, size_t* bytes) {
+  if (!bytes) return EBSD_ERR_NULL;
+  if (m < 1 || n < 1 || splits < 1) return EBSD_ERR_SHAPE;
+  *bytes = (size_t)splits * m * n * sizeof(float);
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_gemm_nt_splitk(const void* a, int m, const void* b, int n, int ld, int dtype, float* c,
+                                   int ldc, float alpha, float beta, int splits, void* workspace,
+                                   size_t ws_bytes, void* stream) {
+  if (!a || !b || !c) return EBSD_ERR_NULL;
+  if (m < 1 || n < 1 || ld < 1 || ldc < n || splits < 1) return EBSD_ERR_SHAPE;
+  const int eb = elem_bytes(dtype);
+  if (eb == 0) return EBSD_ERR_DTYPE;
+  if (ld % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
+  if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) != 0) return EBSD_ERR_ALIGN;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int stages = ld * eb / STAGE_BYTES;
+  const int per = (stages + splits - 1) / splits;
+  const int used = (stages + per - 1) / per;
+  if (used <= 1)
+    return launch_gemm(a, m, b, n, ld, dtype, c, ldc, alpha, beta, nullptr, nullptr, 0.0f, 0, st);
+  if (!workspace || ws_bytes < (size_t)used * m * n * sizeof(float)) return EBSD_ERR_WORKSPACE;
+  float* part = static_cast<float*>(workspace);
+  const int r = launch_gemm(a, m, b, n, ld, dtype, part, n, 1.0f, 0.0f, nullptr, nullptr, 0.0f, 0, st, used, per,
+                            (int64_t)m * n);
+  if (r) return r;
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)(((int64_t)m * n + 255) / 256)), dim3(256), 0, st, part,
+                     used, m, n, c, ldc, alpha, beta);
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
 }
 
 extern "C" int ebsd_cov_update_workspace(int n, int d, size_t* bytes) {

@@ -198,7 +198,7 @@ template <int ITEMS, typename OutT>
 __global__ __launch_bounds__(256) void project_kernel(
     const float* __restrict__ mp, int H, int W, const float* __restrict__ rays, int n_pix,
     const float* __restrict__ quats, int n_ori, OutT* __restrict__ out, int ld,
-    float* __restrict__ out_scale, int flags) {
+    float* __restrict__ out_scale, int flags, const float* __restrict__ offsets) {
   __shared__ float red[4];
   constexpr bool kRaysInRegs = ITEMS <= 16;
   const bool renorm = (flags & EBSD_FLAG_RENORM_DIRS) != 0;
@@ -215,6 +215,14 @@ __global__ __launch_bounds__(256) void project_kernel(
   }
   for (int n = blockIdx.x; n < n_ori; n += gridDim.x) {
     const float4 q = *reinterpret_cast<const float4*>(quats + 4 * (size_t)n);
+    // per-pattern shift of the (un-normalised) pixel positions: a moved
+    // pattern centre translates every detector pixel by the same vector
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    if (offsets) {
+      ox = offsets[3 * (size_t)n];
+      oy = offsets[3 * (size_t)n + 1];
+      oz = offsets[3 * (size_t)n + 2];
+    }
     float vals[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -230,6 +238,12 @@ __global__ __launch_bounds__(256) void project_kernel(
           bx = __ldg(rays + 3 * d + 0);
           by = __ldg(rays + 3 * d + 1);
           bz = __ldg(rays + 3 * d + 2);
+        }
+        if (offsets) {
+          bx += ox;
+          by += oy;
+          bz += oz;
+          renormalise(bx, by, bz);
         }
         float x, y, z;
         rotate_ray(q.x, q.y, q.z, q.w, bx, by, bz, x, y, z);
@@ -277,7 +291,8 @@ __global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restric
 
 template <typename OutT>
 int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, const float* quats,
-                   int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st) {
+                   int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st,
+                   const float* offsets = nullptr) {
   // ~16 resident workgroups per CU on 256 CUs; each walks n_ori / grid patterns
   const dim3 grid(n_ori < 4096 ? n_ori : 4096), block(256);
   OutT* o = static_cast<OutT*>(out);
@@ -285,7 +300,7 @@ int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, 
 #define EBSD_PROJ_CASE(I)                                                                        \
   if (items <= I) {                                                                              \
     hipLaunchKernelGGL((project_kernel<I, OutT>), grid, block, 0, st, mp, H, W, rays, n_pix, quats, \
-                       n_ori, o, ld, out_scale, flags);                                                 \
+                       n_ori, o, ld, out_scale, flags, offsets);                                                 \
     EBSD_LAUNCH_CHECK();                                                                         \
     return EBSD_OK;                                                                              \
   }
@@ -361,6 +376,35 @@ extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const flo
       return launch_project<__half>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
     case EBSD_DTYPE_FP8_E4M3:
       return launch_project<uint8_t>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+  }
+  return EBSD_ERR_DTYPE;
+}
+
+extern "C" int ebsd_project_shifted(const float* mp, int mp_rows, int mp_cols, const float* coords,
+                                    int n_pix, const float* quats, const float* offsets, int n_ori, void* out,
+                                    int out_dtype, int ld_out, float* out_scale, int flags, void* stream) {
+  if (!mp || !coords || !quats || !offsets || !out) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols < 1 || n_ori < 0) return EBSD_ERR_SHAPE;
+  if ((int64_t)mp_rows * mp_cols > (int64_t)INT_MAX) return EBSD_ERR_SHAPE;
+  const int chk = check_ld(out_dtype, n_pix, ld_out);
+  if (chk) return chk;
+  if (out_dtype == EBSD_DTYPE_FP8_E4M3 && !out_scale) return EBSD_ERR_NULL;
+  if ((reinterpret_cast<uintptr_t>(quats) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (n_ori == 0) return EBSD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (out_dtype) {
+    case EBSD_DTYPE_F32:
+      return launch_project<float>(mp, mp_rows, mp_cols, coords, n_pix, quats, n_ori, out, ld_out, out_scale,
+                                   flags, st, offsets);
+    case EBSD_DTYPE_BF16:
+      return launch_project<__hip_bfloat16>(mp, mp_rows, mp_cols, coords, n_pix, quats, n_ori, out, ld_out,
+                                            out_scale, flags, st, offsets);
+    case EBSD_DTYPE_F16:
+      return launch_project<__half>(mp, mp_rows, mp_cols, coords, n_pix, quats, n_ori, out, ld_out, out_scale,
+                                    flags, st, offsets);
+    case EBSD_DTYPE_FP8_E4M3:
+      return launch_project<uint8_t>(mp, mp_rows, mp_cols, coords, n_pix, quats, n_ori, out, ld_out, out_scale,
+                                     flags, st, offsets);
   }
   return EBSD_ERR_DTYPE;
 }

@@ -83,6 +83,21 @@ int ebsd_project(const float* mp, int mp_rows, int mp_cols, const float* rays,
                  void* stream);
 
 /*
+ * ebsd_project with a per-orientation pattern-centre shift: ray d of row n
+ * is normalise(coords[d] + offsets[n]) before the rotation. coords are the
+ * un-normalised sample-frame pixel positions, offsets (n_ori, 3) f32 the
+ * translation a moved pattern centre applies to every pixel (the derivative
+ * of geometry.py:75-89 with respect to the pattern centre times the shift).
+ * Used to simulate training patterns with a jittered pattern centre.
+ * No counterpart in the reference (the learned-proposal form's simulator).
+ */
+int ebsd_project_shifted(const float* mp, int mp_rows, int mp_cols,
+                         const float* coords, int n_pix, const float* quats,
+                         const float* offsets, int n_ori, void* out,
+                         int out_dtype, int ld_out, float* out_scale, int flags,
+                         void* stream);
+
+/*
  * Master-pattern lookup at arbitrary directions: out[i] = bilinear sample
  * (border, align_corners=False) of the side-by-side Lambert master pattern at
  * direction dirs[i] (re-normalised first when renorm != 0).
@@ -144,6 +159,19 @@ int ebsd_rescore(const float* q, int m, const float* dict, int n, int ld, const 
                  int n_cand, int topk, float* out_val, int* out_idx, void* stream);
 
 /*
+ * Detector noise: out[n, d] = Poisson(mean_counts[n] * clean[n, d] /
+ * mean_d(clean[n, :])) / mean_counts[n] + sigma * N(0, 1), for the first n_pix
+ * columns of each row; Philox4x32-10 counter-based random numbers keyed by
+ * seed, counted by (offset, row, column), so the result does not depend on
+ * the launch. No counterpart in the reference (synthetic workload and
+ * learned-proposal training data).
+ */
+int ebsd_poisson_gaussian_noise(const float* clean, int n, int n_pix, int ld_in,
+                                float* out, int ld_out, const float* mean_counts,
+                                float sigma, unsigned long long seed,
+                                unsigned long long offset, void* stream);
+
+/*
  * Dense product C = alpha * A * B^T + beta * C (FP32 accumulation).
  *   a (m, ld), b (n, ld) of dtype F32 / BF16 / F16, row-major, ld a multiple
  *   of ebsd_pad_multiple(dtype) with zero padding; c (m, ldc) f32.
@@ -152,6 +180,18 @@ int ebsd_rescore(const float* q, int m, const float* dict, int n, int ld, const 
  */
 int ebsd_gemm_nt(const void* a, int m, const void* b, int n, int ld, int dtype,
                  float* c, int ldc, float alpha, float beta, void* stream);
+
+/*
+ * ebsd_gemm_nt with the K loop split `splits` ways into partial products
+ * (workspace: splits * m * n floats, sized by ebsd_gemm_nt_splitk_workspace)
+ * that are then summed in a fixed order: for long reductions with small
+ * outputs (weight gradients). No counterpart in the reference.
+ */
+int ebsd_gemm_nt_splitk_workspace(int m, int n, int splits, size_t* bytes);
+int ebsd_gemm_nt_splitk(const void* a, int m, const void* b, int n, int ld,
+                        int dtype, float* c, int ldc, float alpha, float beta,
+                        int splits, void* workspace, size_t workspace_bytes,
+                        void* stream);
 
 /* Bytes of scratch that ebsd_cov_update needs for a batch of n rows of d. */
 int ebsd_cov_update_workspace(int n, int d, size_t* bytes);
@@ -170,6 +210,45 @@ int ebsd_cov_update_workspace(int n, int d, size_t* bytes);
 int ebsd_cov_update(const float* x, int n, int d, int ld_x, float* mean,
                     float* cov, int ld_cov, long long n_seen, void* workspace,
                     size_t workspace_bytes, void* stream);
+
+/*
+ * Learned orientation proposal: pieces of the convolutional network's forward
+ * and backward passes (3x3 stride-2 padding-1 convolutions on NHWC float32
+ * activations run as im2col + ebsd_gemm_nt). None of these has a counterpart
+ * in the reference, which has no network; they implement the north-star
+ * "learned proposal" form whose supports are orientations.py:940-963 (6D ->
+ * rotation) and laue_generators.py:54-249 (symmetry operators).
+ *   ebsd_nn_im2col     x (b, h, w, c) f32 -> col (b*ho*wo, ld) F32/BF16,
+ *                      column (ky*3 + kx)*c + ch, zero padding to ld
+ *   ebsd_nn_col2im     dcol (b*ho*wo, ld) f32 -> dx (b, h, w, c) f32 (gather)
+ *   ebsd_nn_bias_act   y[r, :cols] = act(y + bias), act = ReLU if relu
+ *   ebsd_nn_relu_bwd   dy *= (y > 0), n elements
+ *   ebsd_nn_colsum     out[c] = sum_r x[r, c] (deterministic two-level sum)
+ *   ebsd_nn_transpose  dst (cols, ld_dst) = src^T (F32/BF16), zero rows beyond
+ *   ebsd_nn_geodesic_loss  per sample: 6D -> R (Gram-Schmidt), loss = min over
+ *                      the n_ops symmetry rotations G of the geodesic angle
+ *                      between R and G R(q_true); grad = d(mean loss)/d(6D)
+ *   ebsd_nn_adam       Adam step with bias correction on FP32 weights, also
+ *                      writing a BF16 copy when w_bf16 != NULL
+ */
+int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, void* col, int ld,
+                   int dtype, void* stream);
+int ebsd_nn_col2im(const float* dcol, int ld, int b, int h, int w, int c,
+                   float* dx, void* stream);
+int ebsd_nn_bias_act(float* y, int rows, int cols, int ld, const float* bias,
+                     int relu, void* stream);
+int ebsd_nn_relu_bwd(float* dy, const float* y, long long n, void* stream);
+int ebsd_nn_colsum_workspace(int rows, int cols, size_t* bytes);
+int ebsd_nn_colsum(const float* x, int rows, int cols, int ld, float* out,
+                   void* workspace, size_t workspace_bytes, void* stream);
+int ebsd_nn_transpose(const float* src, int rows, int cols, int ld_src,
+                      void* dst, int ld_dst, int dtype, void* stream);
+int ebsd_nn_geodesic_loss(const float* z6, int ldz, int b, const float* q_true,
+                          const float* ops, int n_ops, float* loss, float* grad,
+                          int ldg, void* stream);
+int ebsd_nn_adam(float* w, const float* g, float* m, float* v, long long n,
+                 float lr, float beta1, float beta2, float eps, int step,
+                 float weight_decay, void* w_bf16, void* stream);
 
 #ifdef __cplusplus
 }
