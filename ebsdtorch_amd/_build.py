@@ -25,6 +25,7 @@ SOURCES = {
     "pca.hip": [],
     "noise.hip": [],
     "nn.hip": [],
+    "local.hip": [],
 }
 
 COMMON_FLAGS = [

@@ -1,0 +1,218 @@
+// Local-dictionary refinement for the learned-proposal form (gfx950).
+//
+// For query pattern i with proposal quaternion c_i and a fixed set of small
+// rotations g_j (a local orientation grid, ebsd/refining.py:13-68, composed
+// as qu_prod(g_j, c_i) like ebsd/refining.py:114), every candidate pattern
+// p_ij(d) = MP(lambert(R(g_j c_i) ray_d)) is generated on the fly and scored
+// against the query x_i by the cosine of the zero-mean vectors
+//   (sum x p - sum p * sum x / D) / (|x| sqrt(sum p^2 - (sum p)^2 / D)),
+// with |x| taken as 1 (the caller passes unit-norm queries). Only the best
+// `topk` (score, j) per query are written: candidate patterns never touch
+// memory. The work is D bilinear samples per candidate, gather/VALU bound.
+//
+// Layout: one workgroup per query; each lane owns one candidate and walks
+// all D pixels, so the per-pixel data (ray, query value) is wave-uniform and
+// comes through scalar loads, and no reduction crosses lanes. A wave's 64
+// candidates are neighbours in the grid (the host orders the grid along a
+// space-filling curve), so for one pixel the 64 lanes sample a compact patch
+// of the master pattern and the gathers share cache lines. The master
+// pattern is read as a quad table: texel (y, x) holds the four bilinear taps
+// (y, x), (y, x+1), (y+1, x), (y+1, x+1) (indices clamped to the image like
+// the border lookup), so a sample is one 16-byte load.
+//
+// Sums are taken about the lane's first sample p0 (sum of p - p0 etc.; the
+// cosine is unchanged by the shift) in four interleaved partial sums, which
+// keeps the variance term free of cancellation.
+
+#include "common.h"
+#include "lambert.h"
+
+namespace {
+
+using namespace ebsd_lambert;
+
+constexpr int kLocalWaves = 4;
+constexpr int kMaxGrid = 4096;
+
+struct Rot {
+  float m[9];
+};
+
+// Active rotation matrix of q = a * b (Hamilton product, w first), after
+// normalising q; v' = R v equals the vector part of q (0, v) q*.
+__device__ __forceinline__ Rot compose_matrix(const float* __restrict__ a, const float* __restrict__ b) {
+  float w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  float x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+  float y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+  float z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+  const float inv = rsqrtf(w * w + x * x + y * y + z * z);
+  w *= inv;
+  x *= inv;
+  y *= inv;
+  z *= inv;
+  Rot r;
+  r.m[0] = 1.0f - 2.0f * (y * y + z * z);
+  r.m[1] = 2.0f * (x * y - w * z);
+  r.m[2] = 2.0f * (x * z + w * y);
+  r.m[3] = 2.0f * (x * y + w * z);
+  r.m[4] = 1.0f - 2.0f * (x * x + z * z);
+  r.m[5] = 2.0f * (y * z - w * x);
+  r.m[6] = 2.0f * (x * z - w * y);
+  r.m[7] = 2.0f * (y * z + w * x);
+  r.m[8] = 1.0f - 2.0f * (x * x + y * y);
+  return r;
+}
+
+// Bilinear border lookup (align_corners=False) through the quad table.
+__device__ __forceinline__ float sample_quad(const float4* __restrict__ quads, int H, int W, float gx, float gy) {
+  float ix = fmaf(gx + 1.0f, 0.5f * (float)W, -0.5f);
+  float iy = fmaf(gy + 1.0f, 0.5f * (float)H, -0.5f);
+  ix = fminf(fmaxf(ix, 0.0f), (float)(W - 1));
+  iy = fminf(fmaxf(iy, 0.0f), (float)(H - 1));
+  const float x0f = floorf(ix);
+  const float y0f = floorf(iy);
+  const float w = ix - x0f;
+  const float n = iy - y0f;
+  const float4 t = quads[(int)y0f * W + (int)x0f];
+  const float top = fmaf(w, t.y - t.x, t.x);
+  const float bot = fmaf(w, t.w - t.z, t.z);
+  return fmaf(n, bot - top, top);
+}
+
+__device__ __forceinline__ float sample_candidate(const float4* __restrict__ quads, int H, int W, const Rot& R,
+                                                  float rx, float ry, float rz) {
+  const float vx = fmaf(R.m[0], rx, fmaf(R.m[1], ry, R.m[2] * rz));
+  const float vy = fmaf(R.m[3], rx, fmaf(R.m[4], ry, R.m[5] * rz));
+  const float vz = fmaf(R.m[6], rx, fmaf(R.m[7], ry, R.m[8] * rz));
+  float gx, gy;
+  lambert_sbs(vx, vy, vz, gx, gy);
+  return sample_quad(quads, H, W, gx, gy);
+}
+
+__global__ __launch_bounds__(256) void quad_table_kernel(const float* __restrict__ mp, int H, int W,
+                                                         float4* __restrict__ quads) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)H * W) return;
+  const int y = (int)(idx / W), x = (int)(idx % W);
+  const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
+  quads[idx] = make_float4(mp[y * W + x], mp[y * W + x1], mp[y1 * W + x], mp[y1 * W + x1]);
+}
+
+__global__ __launch_bounds__(kLocalWaves * 64) void local_refine_kernel(
+    const float4* __restrict__ quads, int H, int W, const float* __restrict__ rays, int n_pix,
+    const float* __restrict__ Q, int ld, const float* __restrict__ centers, const float* __restrict__ grid,
+    int n_grid, int topk, float* __restrict__ out_val, int* __restrict__ out_idx) {
+  __shared__ float scores[kMaxGrid];
+  __shared__ float red[4];
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const float* __restrict__ xrow = Q + (size_t)i * ld;
+
+  float sx = 0.0f;
+  for (int d = tid; d < n_pix; d += kLocalWaves * 64) sx += xrow[d];
+  const float Sx = block_sum_256(sx, red);
+  const float inv_d = 1.0f / (float)n_pix;
+
+  float c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] = centers[(size_t)i * 4 + k];
+
+  const int n_chunks = (n_grid + 63) / 64;
+  for (int ch = wave; ch < n_chunks; ch += kLocalWaves) {
+    const int j = ch * 64 + lane;
+    const Rot R = compose_matrix(grid + (size_t)min(j, n_grid - 1) * 4, c);
+    const float p0 = sample_candidate(quads, H, W, R, rays[0], rays[1], rays[2]);
+    float sp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float spp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float sxp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int d = 0;
+    for (; d + 4 <= n_pix; d += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float* r = rays + 3 * (d + u);
+        const float p = sample_candidate(quads, H, W, R, r[0], r[1], r[2]) - p0;
+        sp[u] += p;
+        spp[u] = fmaf(p, p, spp[u]);
+        sxp[u] = fmaf(xrow[d + u], p, sxp[u]);
+      }
+    }
+    for (; d < n_pix; ++d) {
+      const float* r = rays + 3 * d;
+      const float p = sample_candidate(quads, H, W, R, r[0], r[1], r[2]) - p0;
+      sp[0] += p;
+      spp[0] = fmaf(p, p, spp[0]);
+      sxp[0] = fmaf(xrow[d], p, sxp[0]);
+    }
+    const float Sp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+    const float Spp = (spp[0] + spp[1]) + (spp[2] + spp[3]);
+    const float Sxp = (sxp[0] + sxp[1]) + (sxp[2] + sxp[3]);
+    const float var = Spp - Sp * Sp * inv_d;
+    const float num = Sxp - Sp * Sx * inv_d;
+    if (j < n_grid) scores[j] = var > 0.0f ? num * rsqrtf(var) : 0.0f;
+  }
+  __syncthreads();
+
+  // top-k by repeated wave arg-max over the LDS scores (higher score, then
+  // lower index); a chosen entry is retired by setting it to -inf.
+  if (wave == 0) {
+    for (int k = 0; k < topk; ++k) {
+      float bv = -INFINITY;
+      int bi = INT_MAX;
+      for (int j = lane; j < n_grid; j += 64) {
+        const float v = scores[j];
+        if (topk_better(v, j, bv, bi)) {
+          bv = v;
+          bi = j;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (topk_better(ov, oi, bv, bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      const bool found = bi != INT_MAX && bv != -INFINITY;
+      if (lane == 0) {
+        out_val[(size_t)i * topk + k] = found ? bv : -INFINITY;
+        out_idx[(size_t)i * topk + k] = found ? bi : -1;
+      }
+      if (found && (bi & 63) == lane) scores[bi] = -INFINITY;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, float* quads, void* stream) {
+  if (!mp || !quads) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols > INT_MAX) return EBSD_ERR_SHAPE;
+  if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
+  const int64_t n = (int64_t)mp_rows * mp_cols;
+  hipLaunchKernelGGL(quad_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), mp, mp_rows, mp_cols, reinterpret_cast<float4*>(quads));
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_local_refine(const float* quads, int mp_rows, int mp_cols, const float* rays, int n_pix,
+                                 const float* q, int m, int ld, const float* centers, const float* grid,
+                                 int n_grid, int topk, float* out_val, int* out_idx, void* stream) {
+  if (!quads || !rays || !q || !centers || !grid || !out_val || !out_idx) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols > INT_MAX) return EBSD_ERR_SHAPE;
+  if (n_pix < 1 || ld < n_pix || m < 0 || n_grid < 1 || n_grid > kMaxGrid) return EBSD_ERR_SHAPE;
+  if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
+  if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (m == 0) return EBSD_OK;
+  hipLaunchKernelGGL(local_refine_kernel, dim3(m), dim3(kLocalWaves * 64), 0, static_cast<hipStream_t>(stream),
+                     reinterpret_cast<const float4*>(quads), mp_rows, mp_cols, rays, n_pix, q, ld, centers, grid,
+                     n_grid, topk, out_val, out_idx);
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}

@@ -21,18 +21,21 @@ __device__ __forceinline__ float cast_out<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ __hip_bfloat16 cast_out<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
 
-// col[(b*Ho + oy)*Wo + ox][(ky*3 + kx)*C + c] = X[b][2 oy + ky - 1][2 ox + kx - 1][c]
-// (zero outside the image and in the padding columns [9C, ld)).
-template <typename T>
+// col[(b*Ho + oy)*Wo + ox][(ky*3 + kx)*C + c] = scale * X[b][2 oy + ky - 1][2 ox + kx - 1][c]
+// (zero outside the image and in the padding columns [9C, ld)). With
+// TRANS the matrix is stored transposed, colT[k][row] with rows padded to ld
+// (the layout of the weight-gradient product, whose reduction runs over rows).
+template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void im2col_kernel(const float* __restrict__ X, int B, int H, int W, int C,
-                                                     int Ho, int Wo, T* __restrict__ col, int ld) {
+                                                     int Ho, int Wo, float scale, T* __restrict__ col, int ld) {
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t total = (int64_t)B * Ho * Wo * ld;
+  const int64_t n_rows = (int64_t)B * Ho * Wo;
+  const int64_t total = TRANS ? (int64_t)9 * C * ld : n_rows * ld;
   if (idx >= total) return;
-  const int k = (int)(idx % ld);
-  const int64_t row = idx / ld;
+  const int k = TRANS ? (int)(idx / ld) : (int)(idx % ld);
+  const int64_t row = TRANS ? idx % ld : idx / ld;
   float v = 0.0f;
-  if (k < 9 * C) {
+  if (k < 9 * C && row < n_rows) {
     const int c = k % C;
     const int kk = k / C;
     const int ky = kk / 3, kx = kk % 3;
@@ -40,9 +43,20 @@ __global__ __launch_bounds__(256) void im2col_kernel(const float* __restrict__ X
     const int oy = (int)((row / Wo) % Ho);
     const int b = (int)(row / ((int64_t)Wo * Ho));
     const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
-    if (y >= 0 && y < H && x >= 0 && x < W) v = X[(((int64_t)b * H + y) * W + x) * C + c];
+    if (y >= 0 && y < H && x >= 0 && x < W) v = scale * X[(((int64_t)b * H + y) * W + x) * C + c];
   }
   col[idx] = cast_out<T>(v);
+}
+
+// dst[r][c] = src[r][c] for c < cols, 0 for cols <= c < ld_dst.
+template <typename T>
+__global__ __launch_bounds__(256) void cast_kernel(const float* __restrict__ src, int rows, int cols, int ld_src,
+                                                   T* __restrict__ dst, int ld_dst) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)rows * ld_dst) return;
+  const int c = (int)(idx % ld_dst);
+  const int64_t r = idx / ld_dst;
+  dst[idx] = cast_out<T>(c < cols ? src[r * ld_src + c] : 0.0f);
 }
 
 // dX[b][y][x][c] = sum over the (ky, kx) whose window covers (y, x) of
@@ -230,6 +244,61 @@ __global__ __launch_bounds__(256) void geodesic_loss_kernel(const float* __restr
   go[2] = (g1[2] - g1b1 * b1[2]) / n1;
 }
 
+// 6D -> unit quaternion (w >= 0) of the same rotation the loss uses: rows
+// b1, b2, b3 of R by Gram-Schmidt, then the matrix -> quaternion inverse of
+// quat_matrix by Shepperd's method (largest of 1 + tr, 1 + 2 R_ii - tr
+// chosen as the pivot, so the division is well conditioned).
+__global__ __launch_bounds__(256) void rot6d_quat_kernel(const float* __restrict__ z6, int ldz, int B,
+                                                         float* __restrict__ q) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= B) return;
+  const float* a = z6 + (int64_t)n * ldz;
+  float a1[3] = {a[0], a[1], a[2]}, a2[3] = {a[3], a[4], a[5]};
+  const float n1 = sqrtf(dot3(a1, a1)) + 1e-12f;
+  float b1[3] = {a1[0] / n1, a1[1] / n1, a1[2] / n1};
+  const float p = dot3(b1, a2);
+  float u2[3] = {a2[0] - p * b1[0], a2[1] - p * b1[1], a2[2] - p * b1[2]};
+  const float n2 = sqrtf(dot3(u2, u2)) + 1e-12f;
+  float b2[3] = {u2[0] / n2, u2[1] / n2, u2[2] / n2};
+  float b3[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2], b1[0] * b2[1] - b1[1] * b2[0]};
+  const float R00 = b1[0], R01 = b1[1], R02 = b1[2];
+  const float R10 = b2[0], R11 = b2[1], R12 = b2[2];
+  const float R20 = b3[0], R21 = b3[1], R22 = b3[2];
+  const float tr = R00 + R11 + R22;
+  float w, x, y, z;
+  if (tr >= R00 && tr >= R11 && tr >= R22) {
+    const float s = 2.0f * sqrtf(fmaxf(1.0f + tr, 0.0f));
+    w = 0.25f * s;
+    x = (R21 - R12) / s;
+    y = (R02 - R20) / s;
+    z = (R10 - R01) / s;
+  } else if (R00 >= R11 && R00 >= R22) {
+    const float s = 2.0f * sqrtf(fmaxf(1.0f + 2.0f * R00 - tr, 0.0f));
+    w = (R21 - R12) / s;
+    x = 0.25f * s;
+    y = (R01 + R10) / s;
+    z = (R02 + R20) / s;
+  } else if (R11 >= R22) {
+    const float s = 2.0f * sqrtf(fmaxf(1.0f + 2.0f * R11 - tr, 0.0f));
+    w = (R02 - R20) / s;
+    x = (R01 + R10) / s;
+    y = 0.25f * s;
+    z = (R12 + R21) / s;
+  } else {
+    const float s = 2.0f * sqrtf(fmaxf(1.0f + 2.0f * R22 - tr, 0.0f));
+    w = (R10 - R01) / s;
+    x = (R02 + R20) / s;
+    y = (R12 + R21) / s;
+    z = 0.25f * s;
+  }
+  const float inv = (w < 0.0f ? -1.0f : 1.0f) * rsqrtf(w * w + x * x + y * y + z * z);
+  float* o = q + 4 * (int64_t)n;
+  o[0] = w * inv;
+  o[1] = x * inv;
+  o[2] = y * inv;
+  o[3] = z * inv;
+}
+
 // Adam with bias correction on FP32 master weights; optionally refreshes a
 // BF16 copy used as the GEMM operand.
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
@@ -252,19 +321,43 @@ inline dim3 grid1d(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
-extern "C" int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, void* col, int ld, int dtype,
-                              void* stream) {
+extern "C" int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, float scale, void* col, int ld,
+                              int dtype, int transposed, void* stream) {
   if (!x || !col) return EBSD_ERR_NULL;
-  if (b < 1 || h < 1 || w < 1 || c < 1 || ld < 9 * c) return EBSD_ERR_SHAPE;
+  if (b < 1 || h < 1 || w < 1 || c < 1) return EBSD_ERR_SHAPE;
   const int ho = (h + 1) / 2, wo = (w + 1) / 2;
-  const int64_t total = (int64_t)b * ho * wo * ld;
+  const int64_t n_rows = (int64_t)b * ho * wo;
+  if (transposed ? (ld < n_rows) : (ld < 9 * c)) return EBSD_ERR_SHAPE;
+  const int64_t total = transposed ? (int64_t)9 * c * ld : n_rows * ld;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define EBSD_IM2COL(T, TR)                                                                                   \
+  hipLaunchKernelGGL((im2col_kernel<T, TR>), grid1d(total), dim3(256), 0, st, x, b, h, w, c, ho, wo, scale,  \
+                     static_cast<T*>(col), ld)
+  if (dtype == EBSD_DTYPE_F32) {
+    if (transposed) EBSD_IM2COL(float, true); else EBSD_IM2COL(float, false);
+  } else if (dtype == EBSD_DTYPE_BF16) {
+    if (transposed) EBSD_IM2COL(__hip_bfloat16, true); else EBSD_IM2COL(__hip_bfloat16, false);
+  } else {
+    return EBSD_ERR_DTYPE;
+  }
+#undef EBSD_IM2COL
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_nn_cast(const float* src, int rows, int cols, int ld_src, void* dst, int ld_dst, int dtype,
+                            void* stream) {
+  if (!src || !dst) return EBSD_ERR_NULL;
+  if (rows < 0 || cols < 1 || ld_src < cols || ld_dst < cols) return EBSD_ERR_SHAPE;
+  if (rows == 0) return EBSD_OK;
+  const int64_t total = (int64_t)rows * ld_dst;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == EBSD_DTYPE_F32)
-    hipLaunchKernelGGL(im2col_kernel<float>, grid1d(total), dim3(256), 0, st, x, b, h, w, c, ho, wo,
-                       static_cast<float*>(col), ld);
+    hipLaunchKernelGGL(cast_kernel<float>, grid1d(total), dim3(256), 0, st, src, rows, cols, ld_src,
+                       static_cast<float*>(dst), ld_dst);
   else if (dtype == EBSD_DTYPE_BF16)
-    hipLaunchKernelGGL(im2col_kernel<__hip_bfloat16>, grid1d(total), dim3(256), 0, st, x, b, h, w, c, ho, wo,
-                       static_cast<__hip_bfloat16*>(col), ld);
+    hipLaunchKernelGGL(cast_kernel<__hip_bfloat16>, grid1d(total), dim3(256), 0, st, src, rows, cols, ld_src,
+                       static_cast<__hip_bfloat16*>(dst), ld_dst);
   else
     return EBSD_ERR_DTYPE;
   EBSD_LAUNCH_CHECK();
@@ -366,6 +459,15 @@ extern "C" int ebsd_nn_adam(float* w, const float* g, float* m, float* v, long l
   hipLaunchKernelGGL(adam_kernel, grid1d(n), dim3(256), 0, static_cast<hipStream_t>(stream), w, g, m, v,
                      (int64_t)n, lr, beta1, beta2, eps, c1, c2, weight_decay,
                      static_cast<__hip_bfloat16*>(w_bf16));
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_nn_rot6d_to_quat(const float* z6, int ldz, int b, float* q, void* stream) {
+  if (!z6 || !q) return EBSD_ERR_NULL;
+  if (b < 0 || ldz < 6) return EBSD_ERR_SHAPE;
+  if (b == 0) return EBSD_OK;
+  hipLaunchKernelGGL(rot6d_quat_kernel, grid1d(b), dim3(256), 0, static_cast<hipStream_t>(stream), z6, ldz, b, q);
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;
 }

@@ -18,6 +18,7 @@
 // polynomial (<= 1.44 ulp) evaluated on a reciprocal-based ratio.
 
 #include "common.h"
+#include "lambert.h"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -25,106 +26,9 @@
 
 namespace {
 
+using namespace ebsd_lambert;
+
 constexpr float kFp8Max = 448.0f;  // largest finite OCP e4m3fn value
-
-// Rotate p by unit quaternion (w, x, y, z) with the exact operation order of
-// qu_apply (quaternions.py:203-211): q * (1, p) * conj(q), vector part. Every
-// product and sum rounds separately, as in the reference's elementwise torch
-// ops; a rotation-matrix form is 4x cheaper but moves sampling coordinates by
-// ~3e-7, enough to change values on steep master-pattern bands by ~1e-4.
-__device__ __forceinline__ void rotate_ray(float aw, float ax, float ay, float az,
-                                           float bx, float by, float bz,
-                                           float& ox, float& oy, float& oz) {
-#pragma clang fp contract(off)
-  const float iw = ((aw - ax * bx) - ay * by) - az * bz;
-  const float ix = ((aw * bx + ax) + ay * bz) - az * by;
-  const float iy = ((aw * by - ax * bz) + ay) + az * bx;
-  const float iz = ((aw * bz + ax * by) - ay * bx) + az;
-  ox = (((-iw) * ax + ix * aw) - iy * az) + iz * ay;
-  oy = (((-iw) * ay + ix * az) + iy * aw) - iz * ax;
-  oz = (((-iw) * az - ix * ay) + iy * ax) + iz * aw;
-}
-
-// atan(t) for |t| <= 1: t * P(t^2), an 11-term least-maximum-error fit of
-// atan(t)/t in t^2 on [0, 1] made for this library (max error 1.44 ulp in
-// float32 over [-1, 1], Horner with fused multiply-adds).
-__device__ __forceinline__ float atan_unit(float t) {
-  const float s = t * t;
-  float p = 9.37563132e-04f;
-  p = fmaf(p, s, -6.39479101e-03f);
-  p = fmaf(p, s, 2.04678804e-02f);
-  p = fmaf(p, s, -4.20881540e-02f);
-  p = fmaf(p, s, 6.54138155e-02f);
-  p = fmaf(p, s, -8.71386170e-02f);
-  p = fmaf(p, s, 1.10286514e-01f);
-  p = fmaf(p, s, -1.42744322e-01f);
-  p = fmaf(p, s, 1.99991298e-01f);
-  p = fmaf(p, s, -3.33333021e-01f);
-  p = fmaf(p, s, 9.99999997e-01f);
-  return p * t;
-}
-
-// Side-by-side Rosca-Lambert coordinates in grid_sample's [-1, 1] frame
-// (sphere.py:161-185): gx spans the 2S-wide image (north half left), gy the
-// S rows. The reference's atan2(sml * sgn, |big|) always has
-// |sml| <= |big|, so it is atan of a ratio in [-1, 1].
-__device__ __forceinline__ void lambert_sbs(float x, float y, float z, float& gx, float& gy) {
-#pragma clang fp contract(off)
-  const float factor = __builtin_amdgcn_sqrtf(fmaxf(2.0f * (1.0f - fabsf(z)), 0.0f));
-  const bool cond = fabsf(y) <= fabsf(x);
-  const float big = cond ? x : y;
-  const float sml = cond ? y : x;
-  const float sgn = (big < 0.0f) ? -1.0f : 1.0f;
-  const float sf = sgn * factor;
-  const float abig = fabsf(big);
-  const float ratio = abig > 0.0f ? (sml * sgn) * __builtin_amdgcn_rcpf(abig) : 0.0f;
-  const float simpler = sf * 0.70710678118654752440f;                         // 2 / sqrt(8)
-  const float arct = (sf * atan_unit(ratio)) * 0.90031631615710606956f;      // 2 sqrt(2) / pi
-  const float u = cond ? simpler : arct;
-  gy = cond ? arct : simpler;
-  gx = ((u * 0.5f) - 0.5f) + ((z < 0.0f) ? 1.0f : 0.0f);
-}
-
-// Bilinear sample of an (H, W) image at normalised (gx, gy) with border
-// padding and align_corners=False: source index ((g + 1) * size - 1) / 2,
-// clipped to [0, size - 1]; the four taps are blended as
-// nw*v_nw + ne*v_ne + sw*v_sw + se*v_se (the order of torch's kernel).
-__device__ __forceinline__ float sample_border(const float* __restrict__ img, int H, int W,
-                                               float gx, float gy) {
-#pragma clang fp contract(off)
-  float ix = (gx + 1.0f) * (0.5f * (float)W) - 0.5f;
-  float iy = (gy + 1.0f) * (0.5f * (float)H) - 0.5f;
-  ix = fminf(fmaxf(ix, 0.0f), (float)(W - 1));
-  iy = fminf(fmaxf(iy, 0.0f), (float)(H - 1));
-  const float x0f = floorf(ix);
-  const float y0f = floorf(iy);
-  const int x0 = (int)x0f;
-  const int y0 = (int)y0f;
-  // A tap past the last column/row carries weight exactly 0; clamp its
-  // address so it stays in bounds.
-  const int x1 = min(x0 + 1, W - 1);
-  const int y1 = min(y0 + 1, H - 1);
-  const float w = ix - x0f;
-  const float e = 1.0f - w;
-  const float n = iy - y0f;
-  const float s = 1.0f - n;
-  // 32-bit offsets: the image holds fewer than 2^31 pixels (checked at entry)
-  const float v_nw = __ldg(img + (y0 * W + x0));
-  const float v_ne = __ldg(img + (y0 * W + x1));
-  const float v_sw = __ldg(img + (y1 * W + x0));
-  const float v_se = __ldg(img + (y1 * W + x1));
-  return ((v_nw * (s * e) + v_ne * (s * w)) + v_sw * (n * e)) + v_se * (n * w);
-}
-
-// x / |x| with the squared norm accumulated as fma(z, z, fma(y, y, x*x)),
-// the rounding sequence of torch's float32 CPU vector norm over a length-3
-// row (master_pattern.py:166-167).
-__device__ __forceinline__ void renormalise(float& x, float& y, float& z) {
-  const float nr = sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
-  x = x / nr;
-  y = y / nr;
-  z = z / nr;
-}
 
 template <typename T>
 __device__ __forceinline__ T to_out(float v);

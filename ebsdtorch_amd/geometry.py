@@ -138,6 +138,18 @@ class EBSDGeometry(torch.nn.Module):
         pix = torch.stack(torch.meshgrid(i, j, indexing="ij"), -1).view(-1, 2)
         return self.project2sample(pix)
 
+    def pc_jacobian(self) -> torch.Tensor:
+        """(3, 3) float32 J with: moving the pattern centre by dpc moves every
+        un-normalised sample-frame pixel position by J @ dpc. The translation
+        of bruker_geometry_to_SE3 is affine in the pattern centre and its
+        rotation does not depend on it (nor does the se3 tune), so J is exact;
+        it is read off as t(e_k) - t(0) in float64."""
+        tilt = (self.detector_tilt_deg - self.sample_y_tilt_deg).double().cpu()
+        sx = self.sample_x_tilt_deg.double().cpu()
+        pcs = torch.cat([torch.zeros(1, 3, dtype=torch.float64), torch.eye(3, dtype=torch.float64)])
+        _, t = bruker_geometry_to_SE3(pcs, tilt, sx, self.detector_shape)
+        return (t[1:] - t[:1]).T.to(torch.float32).to(self.pc_SE3_matrix.device)
+
     def get_rays(self, binning: Tuple[int, int] = (1, 1)) -> torch.Tensor:
         """Unit sample-frame rays, (D, 3) float32 (indexing.py:96-106)."""
         c = self.get_coords_sample_frame(binning)

@@ -249,3 +249,279 @@ def cov_update(x: PatternMatrix, mean: torch.Tensor, cov: torch.Tensor, n_seen: 
     st = lib.ebsd_cov_update(x.data.data_ptr(), x.rows, d, x.ld, mean.data_ptr(), cov.data_ptr(), cov.stride(0),
                              int(n_seen), ws.data_ptr(), ws.numel(), _lib.stream_ptr(dev))
     check(st, "ebsd_cov_update")
+
+
+def _f32_on_device(t: torch.Tensor, name: str) -> torch.Tensor:
+    require_device(t, name)
+    if t.dtype != torch.float32:
+        raise ValueError(f"{name} must be float32")
+    return t.contiguous()
+
+
+def project_shifted(master_pattern: torch.Tensor, coords: torch.Tensor, quats: torch.Tensor,
+                    offsets: torch.Tensor, dtype: int = DTYPE_F32, zero_mean: bool = True,
+                    unit_norm: bool = False) -> PatternMatrix:
+    """ebsd_project with a per-pattern shift of the un-normalised pixel
+    positions: ray d of pattern n is normalise(coords[d] + offsets[n])."""
+    mp = _f32_on_device(master_pattern, "master_pattern")
+    coords = _f32_on_device(coords, "coords")
+    quats = _f32_on_device(quats, "quats")
+    offsets = _f32_on_device(offsets, "offsets")
+    if coords.dim() != 2 or coords.shape[1] != 3:
+        raise ValueError("coords must be (D, 3)")
+    n, d = quats.shape[0], coords.shape[0]
+    if quats.shape != (n, 4) or offsets.shape != (n, 3):
+        raise ValueError("quats must be (N, 4) and offsets (N, 3)")
+    out = alloc_patterns(n, d, dtype, quats.device)
+    flags = (FLAG_ZERO_MEAN if zero_mean else 0) | (FLAG_UNIT_NORM if unit_norm else 0)
+    st = _lib.load().ebsd_project_shifted(mp.data_ptr(), mp.shape[0], mp.shape[1], coords.data_ptr(), d,
+                                          quats.data_ptr(), offsets.data_ptr(), n, out.data.data_ptr(), dtype,
+                                          out.ld, _lib.ptr(out.scale), flags, _lib.stream_ptr(quats.device))
+    check(st, "ebsd_project_shifted")
+    return out
+
+
+def poisson_gaussian_noise(clean: torch.Tensor, n_pix: int, mean_counts: torch.Tensor, sigma: float, seed: int,
+                           offset: int = 0, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Counter-based detector noise (ebsd_poisson_gaussian_noise) on the
+    first n_pix columns of each row of `clean` (N, >= n_pix) f32."""
+    require_device(clean, "clean")
+    if clean.dtype != torch.float32 or clean.dim() != 2 or clean.stride(1) != 1:
+        raise ValueError("clean patterns must be float32 (N, P) rows")
+    n = clean.shape[0]
+    mc = _f32_on_device(mean_counts, "mean_counts")
+    if mc.shape != (n,):
+        raise ValueError("mean_counts must be (N,)")
+    if out is None:
+        out = torch.empty((n, n_pix), dtype=torch.float32, device=clean.device)
+    if out.dtype != torch.float32 or out.shape[0] != n or out.shape[1] < n_pix or out.stride(1) != 1:
+        raise ValueError("output must be float32 (N, >= n_pix)")
+    st = _lib.load().ebsd_poisson_gaussian_noise(clean.data_ptr(), n, n_pix, clean.stride(0), out.data_ptr(),
+                                                 out.stride(0), mc.data_ptr(), float(sigma),
+                                                 int(seed) & 0xFFFFFFFFFFFFFFFF, int(offset) & 0xFFFFFFFFFFFFFFFF,
+                                                 _lib.stream_ptr(clean.device))
+    check(st, "ebsd_poisson_gaussian_noise")
+    return out
+
+
+_DT_OF = {torch.float32: DTYPE_F32, torch.bfloat16: DTYPE_BF16, torch.float16: DTYPE_F16}
+
+
+def matmul_nt(a: torch.Tensor, b: torch.Tensor, out: torch.Tensor, alpha: float = 1.0, beta: float = 0.0,
+              splits: int = 1) -> torch.Tensor:
+    """out (m, n) = alpha * a b^T + beta * out for row-major a (m, K),
+    b (n, K) of one dtype, K a multiple of pad_multiple(dtype) with zero
+    padding (ebsd_gemm_nt, or ebsd_gemm_nt_splitk when splits > 1)."""
+    if a.dtype != b.dtype or a.dim() != 2 or b.dim() != 2 or a.shape[1] != b.shape[1]:
+        raise ValueError("operands must be 2D with one dtype and one K")
+    if not (a.is_contiguous() and b.is_contiguous()):
+        raise ValueError("operands must be contiguous")
+    dt = _DT_OF.get(a.dtype)
+    if dt is None:
+        raise ValueError(f"unsupported operand dtype {a.dtype}")
+    m, n = a.shape[0], b.shape[0]
+    if out.dtype != torch.float32 or out.shape[0] != m or out.shape[1] < n or out.stride(1) != 1:
+        raise ValueError("output must be float32 (m, >= n) with unit column stride")
+    lib = _lib.load()
+    dev = a.device
+    if splits > 1:
+        nbytes = ctypes.c_size_t(0)
+        check(lib.ebsd_gemm_nt_splitk_workspace(m, n, splits, ctypes.byref(nbytes)), "ebsd_gemm_nt_splitk_workspace")
+        ws = _workspace(dev, nbytes.value)
+        st = lib.ebsd_gemm_nt_splitk(a.data_ptr(), m, b.data_ptr(), n, a.shape[1], dt, out.data_ptr(),
+                                     out.stride(0), alpha, beta, splits, ws.data_ptr(), ws.numel(),
+                                     _lib.stream_ptr(dev))
+        check(st, "ebsd_gemm_nt_splitk")
+    else:
+        st = lib.ebsd_gemm_nt(a.data_ptr(), m, b.data_ptr(), n, a.shape[1], dt, out.data_ptr(), out.stride(0),
+                              alpha, beta, _lib.stream_ptr(dev))
+        check(st, "ebsd_gemm_nt")
+    return out
+
+
+def im2col(x: torch.Tensor, out: torch.Tensor, scale: float = 1.0, transposed: bool = False) -> torch.Tensor:
+    """3x3 / stride 2 / padding 1 patches of NHWC x (B, H, W, C) f32 into
+    out: (B*Ho*Wo, ld) or, transposed, (9C, ld) of float32 / bfloat16."""
+    x = _f32_on_device(x, "x")
+    b, h, w, c = x.shape
+    dt = _DT_OF.get(out.dtype)
+    if dt not in (DTYPE_F32, DTYPE_BF16) or not out.is_contiguous() or out.dim() != 2:
+        raise ValueError("im2col output must be a contiguous float32/bfloat16 matrix")
+    rows = b * ((h + 1) // 2) * ((w + 1) // 2)
+    need = (9 * c, rows) if transposed else (rows, 9 * c)
+    if out.shape[0] != need[0] or out.shape[1] < need[1]:
+        raise ValueError(f"im2col output must be ({need[0]}, >= {need[1]})")
+    st = _lib.load().ebsd_nn_im2col(x.data_ptr(), b, h, w, c, float(scale), out.data_ptr(), out.shape[1], dt,
+                                    1 if transposed else 0, _lib.stream_ptr(x.device))
+    check(st, "ebsd_nn_im2col")
+    return out
+
+
+def col2im(dcol: torch.Tensor, shape: Tuple[int, int, int, int], out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Adjoint of im2col (ebsd_nn_col2im): dcol (B*Ho*Wo, ld) f32 -> (B, H, W, C)."""
+    dcol = _f32_on_device(dcol, "dcol")
+    b, h, w, c = shape
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=dcol.device)
+    if dcol.shape[0] != b * ((h + 1) // 2) * ((w + 1) // 2) or dcol.shape[1] < 9 * c:
+        raise ValueError("dcol does not match the image shape")
+    st = _lib.load().ebsd_nn_col2im(dcol.data_ptr(), dcol.shape[1], b, h, w, c, out.data_ptr(),
+                                    _lib.stream_ptr(dcol.device))
+    check(st, "ebsd_nn_col2im")
+    return out
+
+
+def cast_pad(src: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+    """out (rows, ld) float32/bfloat16 = src (rows, cols) f32, zero columns beyond."""
+    require_device(src, "src")
+    if src.dtype != torch.float32 or src.dim() != 2 or src.stride(1) != 1:
+        raise ValueError("src must be float32 rows")
+    dt = _DT_OF.get(out.dtype)
+    if dt not in (DTYPE_F32, DTYPE_BF16) or not out.is_contiguous() or out.shape[0] != src.shape[0] \
+            or out.shape[1] < src.shape[1]:
+        raise ValueError("cast output must be contiguous (rows, >= cols) float32/bfloat16")
+    st = _lib.load().ebsd_nn_cast(src.data_ptr(), src.shape[0], src.shape[1], src.stride(0), out.data_ptr(),
+                                  out.shape[1], dt, _lib.stream_ptr(src.device))
+    check(st, "ebsd_nn_cast")
+    return out
+
+
+def transpose_pad(src: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+    """out (cols, ld) float32/bfloat16 = src (rows, cols)^T, zero columns
+    rows..ld-1."""
+    require_device(src, "src")
+    if src.dtype != torch.float32 or src.dim() != 2 or src.stride(1) != 1:
+        raise ValueError("src must be float32 rows")
+    dt = _DT_OF.get(out.dtype)
+    rows, cols = src.shape
+    if dt not in (DTYPE_F32, DTYPE_BF16) or not out.is_contiguous() or out.shape[0] != cols or out.shape[1] < rows:
+        raise ValueError("transpose output must be contiguous (cols, >= rows)")
+    st = _lib.load().ebsd_nn_transpose(src.data_ptr(), rows, cols, src.stride(0), out.data_ptr(), out.shape[1], dt,
+                                       _lib.stream_ptr(src.device))
+    check(st, "ebsd_nn_transpose")
+    return out
+
+
+def bias_act(y: torch.Tensor, bias: torch.Tensor, cols: int, relu: bool) -> torch.Tensor:
+    """y[:, :cols] = act(y[:, :cols] + bias) in place."""
+    require_device(y, "y")
+    bias = _f32_on_device(bias, "bias")
+    if y.dtype != torch.float32 or y.dim() != 2 or y.stride(1) != 1 or bias.numel() < cols:
+        raise ValueError("bias_act needs float32 rows and a bias per column")
+    st = _lib.load().ebsd_nn_bias_act(y.data_ptr(), y.shape[0], cols, y.stride(0), bias.data_ptr(),
+                                      1 if relu else 0, _lib.stream_ptr(y.device))
+    check(st, "ebsd_nn_bias_act")
+    return y
+
+
+def relu_bwd(dy: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+    """dy *= (y > 0) in place (contiguous tensors of one shape)."""
+    if dy.shape != y.shape or not (dy.is_contiguous() and y.is_contiguous()) or dy.dtype != torch.float32:
+        raise ValueError("relu_bwd needs contiguous float32 tensors of one shape")
+    st = _lib.load().ebsd_nn_relu_bwd(dy.data_ptr(), y.data_ptr(), dy.numel(), _lib.stream_ptr(dy.device))
+    check(st, "ebsd_nn_relu_bwd")
+    return dy
+
+
+def colsum(x: torch.Tensor, cols: int, out: torch.Tensor) -> torch.Tensor:
+    """out[c] = sum over rows of x[:, c] for c < cols (deterministic order)."""
+    require_device(x, "x")
+    if x.dtype != torch.float32 or x.dim() != 2 or x.stride(1) != 1 or out.numel() < cols:
+        raise ValueError("colsum needs float32 rows")
+    lib = _lib.load()
+    nbytes = ctypes.c_size_t(0)
+    check(lib.ebsd_nn_colsum_workspace(x.shape[0], cols, ctypes.byref(nbytes)), "ebsd_nn_colsum_workspace")
+    ws = _workspace(x.device, nbytes.value)
+    st = lib.ebsd_nn_colsum(x.data_ptr(), x.shape[0], cols, x.stride(0), out.data_ptr(), ws.data_ptr(), ws.numel(),
+                            _lib.stream_ptr(x.device))
+    check(st, "ebsd_nn_colsum")
+    return out
+
+
+def geodesic_loss(z6: torch.Tensor, q_true: torch.Tensor, ops: torch.Tensor, loss: torch.Tensor,
+                  grad: torch.Tensor) -> None:
+    """Symmetric geodesic loss of 6D outputs z6 (B, >= 6) against unit
+    quaternions q_true (B, 4) over symmetry rotations ops (n, 4): per-sample
+    loss (B,) and d(mean loss)/d(z6) into grad (B, >= 6)."""
+    require_device(z6, "z6")
+    q_true = _f32_on_device(q_true, "q_true")
+    ops = _f32_on_device(ops, "ops")
+    b = z6.shape[0]
+    if z6.dtype != torch.float32 or z6.stride(1) != 1 or z6.shape[1] < 6 or q_true.shape != (b, 4):
+        raise ValueError("z6 must be float32 (B, >= 6) and q_true (B, 4)")
+    if loss.shape != (b,) or grad.shape[0] != b or grad.shape[1] < 6 or grad.stride(1) != 1:
+        raise ValueError("loss must be (B,) and grad (B, >= 6)")
+    st = _lib.load().ebsd_nn_geodesic_loss(z6.data_ptr(), z6.stride(0), b, q_true.data_ptr(), ops.data_ptr(),
+                                           ops.shape[0], loss.data_ptr(), grad.data_ptr(), grad.stride(0),
+                                           _lib.stream_ptr(z6.device))
+    check(st, "ebsd_nn_geodesic_loss")
+
+
+def adam_step(w: torch.Tensor, g: torch.Tensor, m: torch.Tensor, v: torch.Tensor, lr: float, beta1: float,
+              beta2: float, eps: float, step: int, weight_decay: float = 0.0,
+              w_bf16: Optional[torch.Tensor] = None) -> None:
+    """One Adam update of the float32 master weights w (and their bfloat16
+    copy) from gradient g with moment buffers m, v (all contiguous)."""
+    for t in (w, g, m, v):
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != w.numel():
+            raise ValueError("Adam buffers must be contiguous float32 of one size")
+    if w_bf16 is not None and (w_bf16.dtype != torch.bfloat16 or w_bf16.numel() != w.numel()
+                               or not w_bf16.is_contiguous()):
+        raise ValueError("the bfloat16 copy must match the weights")
+    st = _lib.load().ebsd_nn_adam(w.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), w.numel(), lr, beta1, beta2,
+                                  eps, int(step), weight_decay, _lib.ptr(w_bf16), _lib.stream_ptr(w.device))
+    check(st, "ebsd_nn_adam")
+
+
+def mp_quad_table(master_pattern: torch.Tensor) -> torch.Tensor:
+    """(S, C) master pattern -> (S, C, 4) float32 bilinear quad table
+    (ebsd_mp_quad_table)."""
+    mp = _f32_on_device(master_pattern, "master_pattern")
+    if mp.dim() != 2:
+        raise ValueError("master pattern must be a 2D side-by-side image")
+    out = torch.empty((mp.shape[0], mp.shape[1], 4), dtype=torch.float32, device=mp.device)
+    st = _lib.load().ebsd_mp_quad_table(mp.data_ptr(), mp.shape[0], mp.shape[1], out.data_ptr(),
+                                        _lib.stream_ptr(mp.device))
+    check(st, "ebsd_mp_quad_table")
+    return out
+
+
+def local_refine(quads: torch.Tensor, rays: torch.Tensor, queries: PatternMatrix, centers: torch.Tensor,
+                 grid: torch.Tensor, topk: int = 1) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Score the local orientations qu_prod(grid[j], centers[i]) of every
+    query against it on the fly (ebsd_local_refine). quads: mp_quad_table of
+    the master pattern; queries: FP32 unit-norm pattern rows. Returns
+    (cosine (N, topk) f32, grid index (N, topk) int32)."""
+    if queries.dtype != DTYPE_F32:
+        raise ValueError("local refinement scores FP32 queries")
+    require_device(quads, "quads")
+    if quads.dtype != torch.float32 or quads.dim() != 3 or quads.shape[2] != 4 or not quads.is_contiguous():
+        raise ValueError("quads must be the (S, C, 4) float32 table of mp_quad_table")
+    rays = _f32_on_device(rays, "rays")
+    centers = _f32_on_device(centers, "centers")
+    grid = _f32_on_device(grid, "grid")
+    n = queries.rows
+    if rays.shape != (queries.n_pix, 3) or centers.shape != (n, 4) or grid.dim() != 2 or grid.shape[1] != 4:
+        raise ValueError("rays (D, 3), centers (N, 4) and grid (G, 4) must match the queries")
+    dev = centers.device
+    val = torch.empty((n, topk), dtype=torch.float32, device=dev)
+    idx = torch.empty((n, topk), dtype=torch.int32, device=dev)
+    st = _lib.load().ebsd_local_refine(quads.data_ptr(), quads.shape[0], quads.shape[1], rays.data_ptr(),
+                                       queries.n_pix, queries.data.data_ptr(), n, queries.ld, centers.data_ptr(),
+                                       grid.data_ptr(), grid.shape[0], topk, val.data_ptr(), idx.data_ptr(),
+                                       _lib.stream_ptr(dev))
+    check(st, "ebsd_local_refine")
+    return val, idx
+
+
+def rot6d_to_quat(z6: torch.Tensor) -> torch.Tensor:
+    """(B, >= 6) network outputs -> (B, 4) unit quaternions, w >= 0, of the
+    Gram-Schmidt rotation the geodesic loss scores."""
+    require_device(z6, "z6")
+    if z6.dtype != torch.float32 or z6.dim() != 2 or z6.shape[1] < 6 or z6.stride(1) != 1:
+        raise ValueError("z6 must be float32 (B, >= 6)")
+    q = torch.empty((z6.shape[0], 4), dtype=torch.float32, device=z6.device)
+    st = _lib.load().ebsd_nn_rot6d_to_quat(z6.data_ptr(), z6.stride(0), z6.shape[0], q.data_ptr(),
+                                           _lib.stream_ptr(z6.device))
+    check(st, "ebsd_nn_rot6d_to_quat")
+    return q

@@ -295,10 +295,58 @@ def disori_angle_laue(q1: torch.Tensor, q2: torch.Tensor, laue_id: int) -> torch
     return 2.0 * torch.acos(c.clamp(-1.0, 1.0))
 
 
-def ori_to_fz_laue(q: torch.Tensor, laue_id: int) -> torch.Tensor:
+def ori_to_fz_laue(q: torch.Tensor, laue_id: int, left: bool = False) -> torch.Tensor:
     """Move orientations into the fundamental zone (laue_fz_ori.py:35-81):
-    choose the symmetry-equivalent q * g with the largest |w|."""
+    choose the symmetry-equivalent q * g with the largest |w|. With
+    left=True the equivalents are g * q instead: the same zone (the real part
+    of g * q equals that of q * g) but the representative whose detector
+    pattern equals q's, since a pattern is unchanged when its orientation is
+    multiplied by a crystal symmetry on the left."""
     ops = laue_elements(laue_id).to(device=q.device, dtype=q.dtype)
-    eq = qu_prod(q[:, None, :], ops[None])
+    eq = qu_prod(ops[None], q[:, None, :]) if left else qu_prod(q[:, None, :], ops[None])
     best = eq[..., 0].abs().argmax(-1)
     return qu_std(eq[torch.arange(q.shape[0], device=q.device), best])
+
+
+def get_local_orientation_grid(semi_edge_in_degrees: float, kernel_radius_in_steps: int,
+                               axial_grid_dimension: int = 3) -> torch.Tensor:
+    """Cubochoric coordinates (n, 3) float32 of a cube of (2r + 1)^3 points
+    centred on the identity with semi-edge semi_edge_in_degrees * pi^(2/3) /
+    360 (refining.py:13-68); axial_grid_dimension 2 keeps the points with at
+    least one zero coordinate, 1 those with at least two."""
+    a = semi_edge_in_degrees * math.pi ** (2.0 / 3.0) / 360.0
+    t = torch.linspace(-a, a, 2 * kernel_radius_in_steps + 1)
+    cu = torch.stack(torch.meshgrid(t, t, t, indexing="ij"), -1).reshape(-1, 3)
+    if axial_grid_dimension == 3:
+        return cu
+    zeros = (cu == 0).sum(-1)
+    if axial_grid_dimension == 2:
+        return cu[zeros >= 1]
+    if axial_grid_dimension == 1:
+        return cu[zeros >= 2]
+    raise ValueError("axial_grid_dimension must be 1, 2, or 3.")
+
+
+def local_orientation_grid_quats(semi_edge_in_degrees: float, kernel_radius_in_steps: int,
+                                 axial_grid_dimension: int = 3) -> torch.Tensor:
+    """The local grid as unit quaternions (n, 4) float32 (cu2qu, w >= 0)."""
+    cu = get_local_orientation_grid(semi_edge_in_degrees, kernel_radius_in_steps, axial_grid_dimension)
+    return cu2qu(cu.double())
+
+
+def local_ball_grid_quats(radius_deg: float = 3.0, radius_steps: int = 6) -> torch.Tensor:
+    """The local cubochoric grid (get_local_orientation_grid, (2r + 1)^3
+    points) as unit quaternions (n, 4) float32, with the semi-edge chosen so
+    that its outermost shell lies at exactly radius_deg. The cubochoric map
+    sends concentric cubes to concentric balls, so the grid is an
+    equal-volume sampling of the ball of rotations within radius_deg of the
+    identity (r = 6: 2,197 orientations, about 0.37 degree apart for 3
+    degrees)."""
+    target = math.radians(radius_deg)
+    semi = radius_deg
+    for _ in range(6):  # the shell angle is nearly linear in the semi-edge
+        a = semi * math.pi ** (2.0 / 3.0) / 360.0
+        q = cu2qu(torch.tensor([[a, 0.0, 0.0]], dtype=torch.float64))
+        ang = 2.0 * math.atan2(float(q[0, 1:].double().norm()), float(q[0, 0]))
+        semi *= target / ang
+    return cu2qu(get_local_orientation_grid(semi, radius_steps).double()).contiguous()

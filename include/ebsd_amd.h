@@ -215,11 +215,15 @@ int ebsd_cov_update(const float* x, int n, int d, int ld_x, float* mean,
  * Learned orientation proposal: pieces of the convolutional network's forward
  * and backward passes (3x3 stride-2 padding-1 convolutions on NHWC float32
  * activations run as im2col + ebsd_gemm_nt). None of these has a counterpart
- * in the reference, which has no network; they implement the north-star
- * "learned proposal" form whose supports are orientations.py:940-963 (6D ->
+ * in the reference, which has no network; they serve the learned-proposal
+ * form, whose supports in the reference are orientations.py:940-963 (6D ->
  * rotation) and laue_generators.py:54-249 (symmetry operators).
  *   ebsd_nn_im2col     x (b, h, w, c) f32 -> col (b*ho*wo, ld) F32/BF16,
- *                      column (ky*3 + kx)*c + ch, zero padding to ld
+ *                      column (ky*3 + kx)*c + ch, values times scale, zero
+ *                      padding to ld; transposed != 0 stores col^T
+ *                      (9c, ld), ld >= b*ho*wo, zero columns beyond
+ *   ebsd_nn_cast       dst (rows, ld_dst) F32/BF16 = src (rows, cols) f32,
+ *                      zero columns beyond cols
  *   ebsd_nn_col2im     dcol (b*ho*wo, ld) f32 -> dx (b, h, w, c) f32 (gather)
  *   ebsd_nn_bias_act   y[r, :cols] = act(y + bias), act = ReLU if relu
  *   ebsd_nn_relu_bwd   dy *= (y > 0), n elements
@@ -230,9 +234,13 @@ int ebsd_cov_update(const float* x, int n, int d, int ld_x, float* mean,
  *                      between R and G R(q_true); grad = d(mean loss)/d(6D)
  *   ebsd_nn_adam       Adam step with bias correction on FP32 weights, also
  *                      writing a BF16 copy when w_bf16 != NULL
+ *   ebsd_nn_rot6d_to_quat  6D -> the loss's rotation R -> unit quaternion
+ *                      (w >= 0) with quat_matrix(q) = R, (b, 4) f32
  */
-int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, void* col, int ld,
-                   int dtype, void* stream);
+int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, float scale,
+                   void* col, int ld, int dtype, int transposed, void* stream);
+int ebsd_nn_cast(const float* src, int rows, int cols, int ld_src, void* dst,
+                 int ld_dst, int dtype, void* stream);
 int ebsd_nn_col2im(const float* dcol, int ld, int b, int h, int w, int c,
                    float* dx, void* stream);
 int ebsd_nn_bias_act(float* y, int rows, int cols, int ld, const float* bias,
@@ -249,6 +257,39 @@ int ebsd_nn_geodesic_loss(const float* z6, int ldz, int b, const float* q_true,
 int ebsd_nn_adam(float* w, const float* g, float* m, float* v, long long n,
                  float lr, float beta1, float beta2, float eps, int step,
                  float weight_decay, void* w_bf16, void* stream);
+int ebsd_nn_rot6d_to_quat(const float* z6, int ldz, int b, float* q, void* stream);
+
+/*
+ * Quad table of a master pattern for ebsd_local_refine: quads (mp_rows,
+ * mp_cols, 4) f32, 16-byte aligned, texel (y, x) = the bilinear taps
+ * mp[y][x], mp[y][x1], mp[y1][x], mp[y1][x1] with x1 = min(x + 1, cols - 1),
+ * y1 = min(y + 1, rows - 1). No counterpart in the reference (a layout of
+ * the master pattern that ebsd/master_pattern.py:197-206 samples).
+ */
+int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, float* quads,
+                       void* stream);
+
+/*
+ * Local-dictionary refinement (learned-proposal form): for each query row i
+ * (unit-norm f32, n_pix values, row stride ld) and each of the n_grid small
+ * rotations g_j, the candidate orientation qu_prod(g_j, centers[i]) is
+ * projected on the fly (rotation, side-by-side Lambert, bilinear border
+ * lookup of the master pattern through its quad table) and scored by the
+ * cosine between the zero-mean candidate pattern and the query; the best
+ * `topk` per query are written sorted descending (ties: lower j), out_idx
+ * holding j (-1 for empty slots). No candidate pattern is stored.
+ *   quads (mp_rows, mp_cols, 4) f32 from ebsd_mp_quad_table
+ *   rays (n_pix, 3) f32; centers (m, 4) f32 proposals
+ *   grid (n_grid, 4) f32 local rotations, n_grid <= 4096
+ *   out_val (m, topk) f32; out_idx (m, topk) int32
+ * Replaces (as a fused kernel): the candidate generation and scoring of
+ * ebsd/refining.py:104-135 (local grid composed with the current orientation,
+ * projected and dotted with the experimental pattern).
+ */
+int ebsd_local_refine(const float* quads, int mp_rows, int mp_cols,
+                      const float* rays, int n_pix, const float* q, int m, int ld,
+                      const float* centers, const float* grid, int n_grid,
+                      int topk, float* out_val, int* out_idx, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,73 @@
+"""Training and indexing with the learned proposal on the GPU: the loss
+falls, a trained network's proposals are accurate enough for the local
+dictionary to find the orientation without the full search, and an
+untrained network's are not (what the network gains over the method without
+it: the full search for every pattern)."""
+
+import math
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+SHAPE = (64, 64)
+
+
+@pytest.fixture(scope="module")
+def world():
+    from ebsdtorch_amd.geometry import EBSDGeometry
+    from ebsdtorch_amd.synthetic import band_master_pattern
+
+    mp = band_master_pattern(201, laue_id=11, n_families=40, sigma_deg=2.0, seed=0).to(DEV)
+    geo = EBSDGeometry(SHAPE, proj_center=(0.5, 0.5, 0.6))
+    return mp, geo
+
+
+def _sim(world, seed):
+    from ebsdtorch_amd.learned import PatternSimulator
+
+    mp, geo = world
+    return PatternSimulator(mp, geo, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.01, seed=seed)
+
+
+def test_training_reduces_loss(world):
+    from ebsdtorch_amd.learned import ProposalNet, train_proposal_net
+
+    net = ProposalNet(SHAPE, DEV, laue_id=11, compute="bf16", seed=1)
+    print("train 300 x 256", flush=True)
+    log = train_proposal_net(net, _sim(world, 1), 300 * 256, batch=256, lr=1e-3, warmup=20, log_every=10)
+    first = sum(v for _, v in log.loss_curve[:3]) / 3
+    last = sum(v for _, v in log.loss_curve[-3:]) / 3
+    print(f"loss {first:.4f} -> {last:.4f} in {log.seconds:.2f} s ({log.patterns / log.seconds:.0f} patterns/s)")
+    assert all(math.isfinite(v) for _, v in log.loss_curve)
+    assert last < 0.8 * first
+
+
+def test_trained_network_gain(world):
+    from ebsdtorch_amd.learned import LearnedIndexer, ProposalNet, train_proposal_net
+    from ebsdtorch_amd.orientations import disori_angle_laue
+
+    mp, geo = world
+    rays = geo.get_rays().to(DEV)
+    trained = ProposalNet(SHAPE, DEV, laue_id=11, compute="bf16", seed=2)
+    untrained = ProposalNet(SHAPE, DEV, laue_id=11, compute="bf16", seed=2)
+    print("train 2000 x 256", flush=True)
+    log = train_proposal_net(trained, _sim(world, 2), 2000 * 256, batch=256, lr=1e-3, warmup=50, log_every=100)
+    print(f"trained on {log.patterns} patterns in {log.seconds:.1f} s, final loss {log.loss_curve[-1][1]:.4f}")
+    x, q_true = _sim(world, 99).batch(1024)
+    res = {}
+    for name, net in (("trained", trained), ("untrained", untrained)):
+        idx = LearnedIndexer(net, mp, rays, 11, fallback=None, threshold=0.0)
+        q, v, j, q0 = idx.local(x)
+        prop_err = torch.rad2deg(disori_angle_laue(q0.double(), q_true.double(), 11))
+        err = torch.rad2deg(disori_angle_laue(q.double(), q_true.double(), 11))
+        flagged = idx.on_shell[j]
+        ok = (err <= 1.0) & ~flagged
+        res[name] = (float(prop_err.median()), float(ok.float().mean()), float(flagged.float().mean()))
+        print(f"{name}: median proposal error {res[name][0]:.2f} deg, refined within 1 deg "
+              f"without fallback {res[name][1]:.3f}, flagged for fallback {res[name][2]:.3f}")
+    assert res["trained"][0] < 10.0 < res["untrained"][0]
+    assert res["trained"][1] > res["untrained"][1] + 0.3
+    assert res["trained"][2] < res["untrained"][2]
