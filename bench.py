@@ -52,6 +52,9 @@ def parse():
     p.add_argument("--agree-sample", type=int, default=16384,
                    help="patterns re-indexed in plain FP32 to measure top-1 agreement")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
+    p.add_argument("--learned-train", type=int, default=2_000_000,
+                   help="training patterns of the learned-proposal form (0 = skip that form)")
+    p.add_argument("--learned-batch", type=int, default=65536, help="patterns indexed by the learned form")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run a 1 GiB device copy (reference kernel for PMC byte counters)")
     return p.parse_args()
@@ -171,6 +174,12 @@ def main():
     proj_bytes = n_dict * (D * out_bytes + 16) + 2 * a.mp_size * a.mp_size * 4 + 12 * D
     proj_gbps = proj_bytes / (proj_kernel_ms / 1000) / 1e9
 
+    learned = None
+    if a.learned_train > 0:
+        del eng
+        torch.cuda.empty_cache()
+        learned = learned_form(a, mp, dico, dev)
+
     cpu = None
     if a.cpu_seconds > 0:
         cpu = cpu_baseline(mp.cpu(), rays.cpu(), exp[:1024].cpu(), dico.cpu(), n_dict, a.cpu_seconds)
@@ -215,9 +224,101 @@ def main():
             "achieved": round(proj_gbps, 1), "peak": PEAKS["hbm_GBps"], "unit": "GB/s",
             "frac": round(proj_gbps / PEAKS["hbm_GBps"], 4), "bytes": proj_bytes,
         },
+        "learned_proposal": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def learned_form(a, mp, dico, dev):
+    """The learned-proposal configuration: 64x64 detector, network trained
+    on `learned_train` patterns simulated on the fly (uniform orientations,
+    Poisson mean 100-400 + N(0, 0.05^2), pattern centre jittered +-0.01),
+    then `learned_batch` noisy patterns at seeded orientations indexed by
+    proposal -> 2,197-orientation local dictionary within 3 degrees -> FP8
+    full-dictionary search (FP32 top-16 rescore) below the calibrated
+    threshold. Compared with plain FP32 DI of the same 1M dictionary on a
+    sample, and with the ground truth (disorientation)."""
+    import torch
+
+    import ebsdtorch_amd as E
+    from ebsdtorch_amd import kernels
+    from ebsdtorch_amd.learned import (LearnedIndexer, PatternSimulator, ProposalNet, calibrate_threshold,
+                                       train_proposal_net)
+    from ebsdtorch_amd.orientations import disori_angle_laue
+
+    shape = (64, 64)
+    geom = E.EBSDGeometry(shape, proj_center=(0.5, 0.5, 0.6))
+    rays = geom.get_rays().to(dev)
+    sim = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.01, seed=5)
+    net = ProposalNet(shape, dev, laue_id=11, compute="bf16", seed=0)
+    tlog = train_proposal_net(net, sim, a.learned_train, batch=512, lr=1e-3, warmup=200, log_every=500)
+    log(f"learned: trained on {tlog.patterns} patterns in {tlog.seconds:.1f}s "
+        f"({tlog.patterns / tlog.seconds:.0f}/s), loss {tlog.loss_curve[0][1]:.3f} -> {tlog.loss_curve[-1][1]:.4f}")
+    fb = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True)
+    idx = LearnedIndexer(net, mp, rays, 11, fallback=fb, threshold=0.0)
+    cal = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=6)
+    idx.threshold = calibrate_threshold(idx, cal, n=8192, max_error_deg=1.0, precision=0.999)
+    test = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=7)
+    n = a.learned_batch
+    x, q_true = test.batch(n)
+    idx.index(x[:4096])  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = idx.index(x)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rate = n / wall
+    # the local-refinement kernel alone (one launch over the batch)
+    xm = kernels.PatternMatrix(x, None, 0, x.shape[1])
+    q0 = net.propose(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    kernels.local_refine(idx.quads, rays, xm, q0, idx.grid, topk=1)
+    e1.record()
+    torch.cuda.synchronize()
+    local_ms = e0.elapsed_time(e1)
+    samples = n * idx.grid.shape[0] * x.shape[1]
+    err = torch.rad2deg(disori_angle_laue(res["orientations"].double(), q_true.double(), 11))
+    qs = torch.tensor([0.5, 0.9, 0.99, 0.999], dtype=torch.float64, device=err.device)
+    quant = [round(v, 3) for v in torch.quantile(err, qs).tolist()]
+    fallback_rate = float(res["fallback"].float().mean())
+    # plain FP32 DI of the same dictionary on a sample: rate and agreement
+    ns = min(8192, n)
+    ref = E.DictionaryIndexer(mp, rays, dico, precision="fp32")
+    ref.index(ref.prepare(x[:1024]), 1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rv, ri = ref.index(ref.prepare(x[:ns]), 1)
+    torch.cuda.synchronize()
+    fp32_rate = ns / (time.perf_counter() - t1)
+    q_fp32 = ref.orientations[ri[:, 0].long()]
+    err_fp32 = torch.rad2deg(disori_angle_laue(q_fp32.double(), q_true[:ns].double(), 11))
+    vs_fp32 = torch.rad2deg(disori_angle_laue(res["orientations"][:ns].double(), q_fp32.double(), 11))
+    del ref, fb
+    torch.cuda.empty_cache()
+    out = {
+        "workload": f"learned proposal: {n} patterns 64x64, local dictionary {idx.grid.shape[0]} within 3 deg, "
+                    f"fallback FP8 DI over {dico.shape[0]}",
+        "patterns_per_s": round(rate, 1),
+        "fp32_di_patterns_per_s": round(fp32_rate, 1),
+        "speedup_vs_fp32_di": round(rate / fp32_rate, 2),
+        "fallback_rate": round(fallback_rate, 4),
+        "threshold": round(idx.threshold, 5),
+        "disorientation_deg_quantiles_vs_truth": dict(zip(["p50", "p90", "p99", "p99.9"], quant)),
+        "frac_within_1deg_of_truth": round(float((err <= 1.0).float().mean()), 5),
+        "fp32_di_frac_within_1deg_of_truth": round(float((err_fp32 <= 1.0).float().mean()), 5),
+        "frac_within_1deg_of_fp32_di": round(float((vs_fp32 <= 1.0).float().mean()), 5),
+        "train": {"patterns": tlog.patterns, "seconds": round(tlog.seconds, 2),
+                  "patterns_per_s": round(tlog.patterns / tlog.seconds, 1),
+                  "loss_first": round(tlog.loss_curve[0][1], 4), "loss_last": round(tlog.loss_curve[-1][1], 4),
+                  "compute": "bf16 operands, fp32 master weights"},
+        "local_refine_kernel": {"ms": round(local_ms, 3), "samples": samples,
+                                "gsamples_per_s": round(samples / (local_ms / 1000) / 1e9, 1)},
+    }
+    log(f"learned: {rate:.0f} patterns/s vs FP32 DI {fp32_rate:.0f}/s, fallback {fallback_rate:.4f}, "
+        f"err quantiles {quant}, local kernel {local_ms:.1f} ms")
+    return out
 
 
 def cpu_baseline(mp, rays, exp, dico, n_dict, seconds):

@@ -474,12 +474,19 @@ class LearnedIndexer:
             local_score[s:s + b] = v
             prop[s:s + b] = q0
             fb[s:s + b] = need
-            if self.fallback is not None:
-                sel = torch.nonzero(need).flatten()
-                if sel.numel():
-                    fv, fi = self._full(p.index_select(0, sel))
-                    ori[s + sel] = self.fallback.orientations[fi.long()]
-                    score[s + sel] = fv
+        # one full-dictionary pass over every pattern that needs it (the
+        # similarity kernel is efficient only on large query batches)
+        if self.fallback is not None:
+            sel = torch.nonzero(fb).flatten()
+            for s in range(0, sel.numel(), 1 << 16):
+                part = sel[s:s + (1 << 16)]
+                fv, fi = self._full(patterns.index_select(0, part.to(patterns.device)).to(dev, torch.float32))
+                # keep the better of the two matches (the local grid is finer
+                # than the dictionary where both found the same basin)
+                better = fv > local_score[part]
+                take = part[better]
+                ori[take] = self.fallback.orientations[fi[better].long()]
+                score[take] = fv[better]
         ori = ori_to_fz_laue(ori, self.laue_id, left=True)
         return {"orientations": ori, "scores": score, "fallback": fb, "proposals": prop,
                 "local_scores": local_score}

@@ -81,22 +81,21 @@ def simulate_patterns(master_pattern: torch.Tensor, rays: torch.Tensor, orientat
                       chunk: int = 1 << 16) -> torch.Tensor:
     """Noisy patterns (N, D) float32 on the GPU: clean projection p, then
     counts ~ Poisson(mean_counts * p / mean(p)), out = counts / mean_counts +
-    N(0, gaussian_sigma^2). mean_counts may be a per-pattern (N,) tensor."""
+    N(0, gaussian_sigma^2) (the counter-based noise kernel, keyed by seed and
+    counted by chunk, row and pixel). mean_counts may be a per-pattern (N,)
+    tensor."""
     dev = orientations.device
-    g = torch.Generator(device=dev).manual_seed(seed)
     n, d = orientations.shape[0], rays.shape[0]
     out = torch.empty((n, d), dtype=torch.float32, device=dev)
     mc = torch.as_tensor(mean_counts, dtype=torch.float32, device=dev)
     if mc.dim() == 0:
         mc = mc.expand(n)
-    for s in range(0, n, chunk):
+    mc = mc.contiguous()
+    for c, s in enumerate(range(0, n, chunk)):
         q = orientations[s:s + chunk]
-        clean = kernels.project(master_pattern, rays, q, zero_mean=False).data[:, :d]
-        m = mc[s:s + chunk, None]
-        rate = m * clean / clean.mean(dim=1, keepdim=True).clamp_min(1e-12)
-        counts = torch.poisson(rate, generator=g)
-        noise = torch.randn(counts.shape, generator=g, device=dev) * gaussian_sigma
-        out[s:s + chunk] = counts / m + noise
+        clean = kernels.project(master_pattern, rays, q, zero_mean=False)
+        kernels.poisson_gaussian_noise(clean.data, d, mc[s:s + chunk].contiguous(), gaussian_sigma, seed=seed,
+                                       offset=c, out=out[s:s + chunk])
     return out
 
 
