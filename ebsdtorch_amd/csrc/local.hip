@@ -24,6 +24,8 @@
 // cosine is unchanged by the shift) in four interleaved partial sums, which
 // keeps the variance term free of cancellation.
 
+#include <algorithm>
+
 #include "common.h"
 #include "lambert.h"
 
@@ -91,11 +93,12 @@ __device__ __forceinline__ float sample_candidate(const float4* __restrict__ qua
 
 __global__ __launch_bounds__(256) void quad_table_kernel(const float* __restrict__ mp, int H, int W,
                                                          float4* __restrict__ quads) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (int64_t)H * W) return;
-  const int y = (int)(idx / W), x = (int)(idx % W);
-  const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
-  quads[idx] = make_float4(mp[y * W + x], mp[y * W + x1], mp[y1 * W + x], mp[y1 * W + x1]);
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < (int64_t)H * W;
+       idx += (int64_t)gridDim.x * 256) {
+    const int y = (int)(idx / W), x = (int)(idx % W);
+    const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
+    quads[idx] = make_float4(mp[y * W + x], mp[y * W + x1], mp[y1 * W + x], mp[y1 * W + x1]);
+  }
 }
 
 __global__ __launch_bounds__(kLocalWaves * 64) void local_refine_kernel(
@@ -195,7 +198,7 @@ extern "C" int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, flo
   if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols > INT_MAX) return EBSD_ERR_SHAPE;
   if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
   const int64_t n = (int64_t)mp_rows * mp_cols;
-  hipLaunchKernelGGL(quad_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(quad_table_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), mp, mp_rows, mp_cols, reinterpret_cast<float4*>(quads));
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;

@@ -28,10 +28,9 @@ __device__ __forceinline__ __hip_bfloat16 cast_out<__hip_bfloat16>(float v) { re
 template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void im2col_kernel(const float* __restrict__ X, int B, int H, int W, int C,
                                                      int Ho, int Wo, float scale, T* __restrict__ col, int ld) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t n_rows = (int64_t)B * Ho * Wo;
   const int64_t total = TRANS ? (int64_t)9 * C * ld : n_rows * ld;
-  if (idx >= total) return;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
   const int k = TRANS ? (int)(idx / ld) : (int)(idx % ld);
   const int64_t row = TRANS ? idx % ld : idx / ld;
   float v = 0.0f;
@@ -46,26 +45,27 @@ __global__ __launch_bounds__(256) void im2col_kernel(const float* __restrict__ X
     if (y >= 0 && y < H && x >= 0 && x < W) v = scale * X[(((int64_t)b * H + y) * W + x) * C + c];
   }
   col[idx] = cast_out<T>(v);
+  }
 }
 
 // dst[r][c] = src[r][c] for c < cols, 0 for cols <= c < ld_dst.
 template <typename T>
 __global__ __launch_bounds__(256) void cast_kernel(const float* __restrict__ src, int rows, int cols, int ld_src,
                                                    T* __restrict__ dst, int ld_dst) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (int64_t)rows * ld_dst) return;
-  const int c = (int)(idx % ld_dst);
-  const int64_t r = idx / ld_dst;
-  dst[idx] = cast_out<T>(c < cols ? src[r * ld_src + c] : 0.0f);
+  const int64_t total = (int64_t)rows * ld_dst;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+    const int c = (int)(idx % ld_dst);
+    const int64_t r = idx / ld_dst;
+    dst[idx] = cast_out<T>(c < cols ? src[r * ld_src + c] : 0.0f);
+  }
 }
 
 // dX[b][y][x][c] = sum over the (ky, kx) whose window covers (y, x) of
 // dcol[(b*Ho + oy)*Wo + ox][(ky*3 + kx)*C + c]; a gather, so no atomics.
 __global__ __launch_bounds__(256) void col2im_kernel(const float* __restrict__ dcol, int ld, int B, int H, int W,
                                                      int C, int Ho, int Wo, float* __restrict__ dX) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t total = (int64_t)B * H * W * C;
-  if (idx >= total) return;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
   const int c = (int)(idx % C);
   const int x = (int)((idx / C) % W);
   const int y = (int)((idx / ((int64_t)C * W)) % H);
@@ -87,26 +87,29 @@ __global__ __launch_bounds__(256) void col2im_kernel(const float* __restrict__ d
     }
   }
   dX[idx] = s;
+  }
 }
 
 // Y[r][c] = relu(Y[r][c] + bias[c]) in place (relu optional).
 __global__ __launch_bounds__(256) void bias_act_kernel(float* __restrict__ Y, int rows, int cols, int ld,
                                                        const float* __restrict__ bias, int relu) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (int64_t)rows * cols) return;
-  const int c = (int)(idx % cols);
-  const int64_t r = idx / cols;
-  float v = Y[r * ld + c] + bias[c];
-  if (relu) v = fmaxf(v, 0.0f);
-  Y[r * ld + c] = v;
+  const int64_t total = (int64_t)rows * cols;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+    const int c = (int)(idx % cols);
+    const int64_t r = idx / cols;
+    float v = Y[r * ld + c] + bias[c];
+    if (relu) v = fmaxf(v, 0.0f);
+    Y[r * ld + c] = v;
+  }
 }
 
 // dZ[r][c] = dY[r][c] * (Y[r][c] > 0) (Y post-activation), in place on dY.
 __global__ __launch_bounds__(256) void relu_bwd_kernel(float* __restrict__ dY, const float* __restrict__ Y,
                                                        int64_t n) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n) return;
-  if (!(Y[idx] > 0.0f)) dY[idx] = 0.0f;
+  const int64_t total = n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+    if (!(Y[idx] > 0.0f)) dY[idx] = 0.0f;
+  }
 }
 
 // out[c] (+)= sum_r X[r][c], two levels: per-chunk partials then a fixed-
@@ -305,19 +308,24 @@ __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ w, const 
                                                    float* __restrict__ m, float* __restrict__ v, int64_t n,
                                                    float lr, float b1, float b2, float eps, float c1, float c2,
                                                    float wd, __hip_bfloat16* __restrict__ w_bf16) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i] + wd * w[i];
-  const float mi = b1 * m[i] + (1.0f - b1) * gi;
-  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  w[i] = wi;
-  if (w_bf16) w_bf16[i] = __float2bfloat16(wi);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const float gi = g[i] + wd * w[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (w_bf16) w_bf16[i] = __float2bfloat16(wi);
+  }
 }
 
-inline dim3 grid1d(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+// 1D launches are grid-stride loops over at most 2^20 workgroups, so any
+// element count fits the launch limits.
+inline dim3 grid1d(int64_t n) {
+  const int64_t blocks = (n + 255) / 256;
+  return dim3((unsigned)(blocks < (1 << 20) ? (blocks > 0 ? blocks : 1) : (1 << 20)));
+}
 
 }  // namespace
 
@@ -441,7 +449,7 @@ extern "C" int ebsd_nn_transpose(const float* src, int rows, int cols, int ld_sr
 extern "C" int ebsd_nn_geodesic_loss(const float* z6, int ldz, int b, const float* q_true, const float* ops,
                                      int n_ops, float* loss, float* grad, int ldg, void* stream) {
   if (!z6 || !q_true || !ops || !loss || !grad) return EBSD_ERR_NULL;
-  if (b < 0 || ldz < 6 || ldg < 6 || n_ops < 1) return EBSD_ERR_SHAPE;
+  if (b < 0 || b > (1 << 28) || ldz < 6 || ldg < 6 || n_ops < 1) return EBSD_ERR_SHAPE;
   if (b == 0) return EBSD_OK;
   hipLaunchKernelGGL(geodesic_loss_kernel, grid1d(b), dim3(256), 0, static_cast<hipStream_t>(stream), z6, ldz, b,
                      q_true, ops, n_ops, loss, grad, ldg);
@@ -465,7 +473,7 @@ extern "C" int ebsd_nn_adam(float* w, const float* g, float* m, float* v, long l
 
 extern "C" int ebsd_nn_rot6d_to_quat(const float* z6, int ldz, int b, float* q, void* stream) {
   if (!z6 || !q) return EBSD_ERR_NULL;
-  if (b < 0 || ldz < 6) return EBSD_ERR_SHAPE;
+  if (b < 0 || b > (1 << 28) || ldz < 6) return EBSD_ERR_SHAPE;
   if (b == 0) return EBSD_OK;
   hipLaunchKernelGGL(rot6d_quat_kernel, grid1d(b), dim3(256), 0, static_cast<hipStream_t>(stream), z6, ldz, b, q);
   EBSD_LAUNCH_CHECK();

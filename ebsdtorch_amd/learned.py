@@ -278,9 +278,13 @@ class ProposalNet:
         kernels.adam_step(self.flat, self.grad, self.adam_m, self.adam_v, lr, beta1, beta2, eps, self.step_count,
                           weight_decay, self.flat_lp)
 
-    def propose(self, x: torch.Tensor) -> torch.Tensor:
-        """(B, H*W) zero-mean unit-norm patterns -> proposals (B, 4), w >= 0."""
-        return kernels.rot6d_to_quat(self.forward(x))
+    def propose(self, x: torch.Tensor, chunk: int = 8192) -> torch.Tensor:
+        """(B, H*W) zero-mean unit-norm patterns -> proposals (B, 4), w >= 0
+        (in chunks of `chunk` patterns, which bounds the im2col buffers)."""
+        out = torch.empty((x.shape[0], 4), dtype=torch.float32, device=self.device)
+        for s in range(0, x.shape[0], chunk):
+            out[s:s + chunk] = kernels.rot6d_to_quat(self.forward(x[s:s + chunk]))
+        return out
 
     # -- reference layout ---------------------------------------------------
     def torch_parameters(self) -> Dict[str, torch.Tensor]:
