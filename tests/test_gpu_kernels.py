@@ -206,3 +206,42 @@ def test_sim_topk_low_precision_exact_on_stored_values(dtype, m, n, d, k):
     idx = torch.empty(m, k, dtype=torch.int32, device=DEV)
     kernels.sim_topk(Q, D, k, val, idx, merge=False)
     _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
+
+
+@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_FP8_E4M3])
+@pytest.mark.parametrize("m,n,d,k", [(5, 256, 384, 16), (1000, 5000, 400, 16), (777, 300, 1000, 1),
+                                     (130, 70000, 520, 8), (4096, 2048, 3600, 16)])
+def test_sim_topk_large_tile_kernel(dtype, m, n, d, k):
+    """Shapes that take the 256 x 128 low-precision kernel (N >= 256, at
+    least three 128-byte K stages): ragged M and N, one and many parts."""
+    _say(f"sim_topk large-tile dtype={dtype} m={m} n={n} d={d} k={k}")
+    g = torch.Generator().manual_seed(m + n + d + k + dtype)
+    q = torch.randn(m, d, generator=g)
+    dd = torch.randn(n, d, generator=g)
+    Q = kernels.prep_patterns(q.to(DEV), dtype, zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), dtype, zero_mean=False)
+    val = torch.empty(m, k, device=DEV)
+    idx = torch.empty(m, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, val, idx, merge=False)
+    _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
+
+
+@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_FP8_E4M3])
+def test_sim_topk_low_precision_streaming_mixes_kernels(dtype):
+    """Chunks below 256 rows take the 128 x 128 kernel, larger ones the
+    256 x 128 kernel; the streamed result equals one pass bit for bit."""
+    _say(f"sim_topk low-precision streaming dtype={dtype}")
+    g = torch.Generator().manual_seed(17 + dtype)
+    q = torch.randn(300, 512, generator=g)
+    dd = torch.randn(3000, 512, generator=g)
+    Q = kernels.prep_patterns(q.to(DEV), dtype, zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), dtype, zero_mean=False)
+    k = 16
+    v1 = torch.empty(300, k, device=DEV)
+    i1 = torch.empty(300, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, v1, i1, merge=False)
+    v2 = torch.full((300, k), float("-inf"), device=DEV)
+    i2 = torch.full((300, k), -1, dtype=torch.int32, device=DEV)
+    for s, c in ((0, 100), (100, 700), (800, 255), (1055, 256), (1311, 1689)):
+        kernels.sim_topk(Q, D.narrow(s, c), k, v2, i2, index_base=s, merge=True)
+    assert torch.equal(i1, i2) and torch.equal(v1, v2)
