@@ -180,6 +180,18 @@ def main():
         torch.cuda.empty_cache()
         learned = learned_form(a, mp, dico, dev)
 
+    # HBM-side bytes per launch of this kernel at this workload, measured
+    # with two PMC passes (FETCH_SIZE, WRITE_SIZE) and committed under
+    # profiles/ (null when the workload differs from the measured one)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sim_topk_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if (t.get("precision") == a.precision and t.get("dictionary") == n_dict and t.get("batch") == a.batch
+                and t.get("detector") == [det, det] and t.get("topk") == a.topk):
+            traffic = t["traffic_bytes"]
+
     cpu = None
     if a.cpu_seconds > 0:
         cpu = cpu_baseline(mp.cpu(), rays.cpu(), exp[:1024].cpu(), dico.cpu(), n_dict, a.cpu_seconds)
@@ -209,13 +221,14 @@ def main():
             None if fp32_rate is None else round(fp32_rate, 1)),
         "speedup_vs_fp32_di": None if fp32_rate is None else round(value / fp32_rate, 2),
         "roofline": {
-            "kernel": "sim_topk (similarity GEMM + fused top-k)",
+            "kernel": "sim_topk (similarity GEMM + fused top-k)" + (
+                " sim_topk_lp_kernel 256x128" if a.precision != "fp32" else " sim_topk_kernel 128x128"),
             "bound": "mfma",
             "achieved": round(achieved, 2),
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": round(sim_avg_ms, 3),
             "flops_per_launch": flops,
         },
