@@ -55,6 +55,10 @@ def parse():
     p.add_argument("--learned-train", type=int, default=2_000_000,
                    help="training patterns of the learned-proposal form (0 = skip that form)")
     p.add_argument("--learned-batch", type=int, default=65536, help="patterns indexed by the learned form")
+    p.add_argument("--workload", default="metric", choices=["metric", "c5"],
+                   help="metric: the headline configuration; c5: 6/mmm, 120x120, 1M dictionary, streamed scan")
+    p.add_argument("--c5-scan", type=int, default=1 << 20, help="c5: patterns in the streamed scan")
+    p.add_argument("--c5-train", type=int, default=2_000_000, help="c5: training patterns of the 6/mmm network")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run a 1 GiB device copy (reference kernel for PMC byte counters)")
     return p.parse_args()
@@ -75,6 +79,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if a.workload == "c5":
+        print(json.dumps(c5_workload(a, dev)), flush=True)
+        return
     if a.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
         dst = torch.empty_like(src)
@@ -332,6 +339,132 @@ def learned_form(a, mp, dico, dev):
     log(f"learned: {rate:.0f} patterns/s vs FP32 DI {fp32_rate:.0f}/s, fallback {fallback_rate:.4f}, "
         f"err quantiles {quant}, local kernel {local_ms:.1f} ms")
     return out
+
+
+def c5_workload(a, dev):
+    """Combined stress case: 6/mmm band master pattern (501x501, 60
+    families), 120x120 detector, ~1M fundamental-zone dictionary, a scan of
+    `c5_scan` noisy patterns (seeded uniform orientations, Poisson mean
+    U(100, 400) + N(0, 0.05^2)) simulated and indexed in 65,536-pattern
+    chunks by (1) FP8 similarity + FP32 top-16 rescore and (2) the learned
+    proposal retrained for 6/mmm with that search as fallback. Cosine
+    ranking (unit-norm dictionary) in both, so the forms share one resident
+    dictionary; FP32 agreement is measured on a 4,096-pattern sample with the
+    FP32 copy of the same dictionary."""
+    import torch
+
+    import ebsdtorch_amd as E
+    from ebsdtorch_amd import kernels, synthetic
+    from ebsdtorch_amd._lib import DTYPE_F32, DTYPE_FP8_E4M3
+    from ebsdtorch_amd.learned import (LearnedIndexer, PatternSimulator, ProposalNet, calibrate_threshold,
+                                       train_proposal_net)
+    from ebsdtorch_amd.orientations import disori_angle_laue
+
+    laue = 9
+    det = 120
+    D = det * det
+    chunk = 65536
+    mp = synthetic.band_master_pattern(501, laue, 60, 2.0, seed=10, device=dev)
+    dico = E.sample_ori_fz_laue(laue, 1_000_000, dev, permute=True, generator=torch.Generator().manual_seed(11))
+    geom = E.EBSDGeometry((det, det), proj_center=(0.5, 0.5, 0.6))
+    rays = geom.get_rays().to(dev)
+    n_dict = dico.shape[0]
+    torch.cuda.synchronize()
+    eng = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True)
+    pe = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    pe[0].record()
+    kernels.project(mp, rays, dico, DTYPE_FP8_E4M3, unit_norm=True, out=eng.dictionary)
+    pe[1].record()
+    kernels.project(mp, rays, dico, DTYPE_F32, unit_norm=True, out=eng.dictionary_f32)
+    pe[2].record()
+    torch.cuda.synchronize()
+    proj8 = pe[0].elapsed_time(pe[1])
+    proj32 = pe[1].elapsed_time(pe[2])
+    bytes8 = n_dict * (eng.dictionary.ld + 4 + 16) + 2 * 501 * 501 * 4 + 12 * D
+    bytes32 = n_dict * (eng.dictionary_f32.ld * 4 + 16) + 2 * 501 * 501 * 4 + 12 * D
+    log(f"c5: dictionary {n_dict} x {D}: projection FP8 {proj8:.1f} ms, FP32 {proj32:.1f} ms")
+
+    sim = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.01, seed=20)
+    net = ProposalNet((det, det), dev, laue_id=laue, compute="bf16", seed=0)
+    tlog = train_proposal_net(net, sim, a.c5_train, batch=256, lr=1e-3, warmup=200, log_every=1000)
+    log(f"c5: trained on {tlog.patterns} patterns in {tlog.seconds:.1f}s, loss {tlog.loss_curve[-1][1]:.4f}")
+    # the local refinement scores 2x2-binned patterns (60x60) against the
+    # 2,197 candidates; the network and the fallback search see 120x120
+    lidx = LearnedIndexer(net, mp, rays, laue, fallback=eng, threshold=0.0, binning=2,
+                          local_rays=geom.get_rays((2, 2)).to(dev))
+    lidx.threshold = calibrate_threshold(
+        lidx, PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=21),
+        n=8192)
+    scan = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=22)
+
+    t_di = t_le = 0.0
+    sim_ms = []
+    err_di, err_le, fb = [], [], []
+    agree = None
+    n_done = 0
+    k = 16
+    while n_done < a.c5_scan:
+        b = min(chunk, a.c5_scan - n_done)
+        x, q_true = scan.batch(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        q8 = kernels.prep_patterns(x, DTYPE_FP8_E4M3, zero_mean=True, unit_norm=True)
+        q32 = kernels.prep_patterns(x, DTYPE_F32, zero_mean=True, unit_norm=True)
+        val = torch.empty((b, k), device=dev)
+        idx = torch.empty((b, k), dtype=torch.int32, device=dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        kernels.sim_topk(q8, eng.dictionary, k, val, idx, merge=False)
+        e1.record()
+        rv, ri = kernels.rescore(q32, eng.dictionary_f32, idx, 1)
+        torch.cuda.synchronize()
+        t_di += time.perf_counter() - t0
+        sim_ms.append((b, e0.elapsed_time(e1)))
+        if agree is None:
+            ns = 4096
+            fv = torch.empty((ns, 1), device=dev)
+            fi = torch.empty((ns, 1), dtype=torch.int32, device=dev)
+            kernels.sim_topk(q32.narrow(0, ns), eng.dictionary_f32, 1, fv, fi, merge=False)
+            agree = float((fi[:, 0] == ri[:ns, 0]).float().mean())
+        t0 = time.perf_counter()
+        res = lidx.index(x)
+        torch.cuda.synchronize()
+        t_le += time.perf_counter() - t0
+        err_di.append(torch.rad2deg(disori_angle_laue(dico[ri[:, 0].long()].double(), q_true.double(), laue)))
+        err_le.append(torch.rad2deg(disori_angle_laue(res["orientations"].double(), q_true.double(), laue)))
+        fb.append(res["fallback"])
+        n_done += b
+        log(f"c5: {n_done}/{a.c5_scan} patterns, FP8 DI {n_done / t_di:.0f}/s, learned {n_done / t_le:.0f}/s")
+    err_di = torch.cat(err_di)
+    err_le = torch.cat(err_le)
+    fb = torch.cat(fb)
+    qs = torch.tensor([0.5, 0.9, 0.99, 0.999], dtype=torch.float64, device=err_di.device)
+    full = [m for nb, m in sim_ms if nb == sim_ms[0][0]]
+    sim_avg = sum(full) / len(full)
+    flops = 2.0 * sim_ms[0][0] * n_dict * D
+    return {
+        "metric": "c5: patterns indexed/s, 6/mmm 1M dictionary, 120x120, streamed scan",
+        "workload": f"6/mmm {n_dict} dictionary x {a.c5_scan}-pattern scan in {chunk}-pattern chunks, {det}x{det}",
+        "n_gpus": 1,
+        "fp8_di": {"patterns_per_s": round(a.c5_scan / t_di, 1), "top1_agreement_vs_fp32": agree,
+                   "agreement_sample": 4096,
+                   "disorientation_deg_quantiles_vs_truth": dict(zip(["p50", "p90", "p99", "p99.9"],
+                                                                     [round(v, 3) for v in torch.quantile(err_di, qs).tolist()])),
+                   "frac_within_1deg_of_truth": round(float((err_di <= 1.0).float().mean()), 5)},
+        "similarity_kernel": {"avg_launch_ms": round(sim_avg, 2), "achieved_TFLOPs": round(flops / sim_avg / 1e9, 1),
+                              "peak_TFLOPs": PEAKS["fp8"], "mfma_frac": round(flops / sim_avg / 1e9 / PEAKS["fp8"], 4)},
+        "projection": {"fp8_ms": round(proj8, 2), "fp8_GBps": round(bytes8 / proj8 / 1e6, 1),
+                       "fp32_ms": round(proj32, 2), "fp32_GBps": round(bytes32 / proj32 / 1e6, 1),
+                       "hbm_peak_GBps": PEAKS["hbm_GBps"]},
+        "learned_proposal": {"patterns_per_s": round(a.c5_scan / t_le, 1), "fallback_rate": round(float(fb.float().mean()), 4),
+                             "local_binning": 2,
+                             "threshold": round(lidx.threshold, 5),
+                             "disorientation_deg_quantiles_vs_truth": dict(zip(["p50", "p90", "p99", "p99.9"],
+                                                                               [round(v, 3) for v in torch.quantile(err_le, qs).tolist()])),
+                             "frac_within_1deg_of_truth": round(float((err_le <= 1.0).float().mean()), 5),
+                             "train": {"patterns": tlog.patterns, "seconds": round(tlog.seconds, 1),
+                                       "loss_last": round(tlog.loss_curve[-1][1], 4)}},
+    }
 
 
 def cpu_baseline(mp, rays, exp, dico, n_dict, seconds):

@@ -413,8 +413,22 @@ class LearnedIndexer:
 
     def __init__(self, net: ProposalNet, master_pattern: torch.Tensor, rays: torch.Tensor, laue_id: int,
                  fallback=None, threshold: float = 0.0, grid: Optional[torch.Tensor] = None,
-                 radius_deg: float = 3.0, radius_steps: int = 6, shell_fallback: bool = True):
+                 radius_deg: float = 3.0, radius_steps: int = 6, shell_fallback: bool = True,
+                 binning: int = 1, local_rays: Optional[torch.Tensor] = None):
         self.net = net
+        # The local refinement may score virtually binned patterns
+        # (average-pooled binning x binning, as ebsd/refining.py's
+        # virtual_binning with get_patterns(binning=...)) against the binned
+        # detector's rays; the network and the fallback see full patterns.
+        self.binning = int(binning)
+        if self.binning > 1:
+            if local_rays is None:
+                raise ValueError("binning > 1 needs the binned detector's rays (geometry.get_rays((b, b)))")
+            if net.H % self.binning or net.W % self.binning:
+                raise ValueError("the binning must divide the detector shape")
+            if local_rays.shape[0] != (net.H // self.binning) * (net.W // self.binning):
+                raise ValueError("local_rays do not match the binned detector")
+        self.local_rays = (rays if self.binning == 1 else local_rays).to(net.device, torch.float32).contiguous()
         self.device = net.device
         self.mp = master_pattern.to(self.device, torch.float32).contiguous()
         self.rays = rays.to(self.device, torch.float32).contiguous()
@@ -441,12 +455,20 @@ class LearnedIndexer:
         if fallback is not None and fallback.n_pix != self.rays.shape[0]:
             raise ValueError("the fallback dictionary must use the same detector pixels")
 
+    def _local_queries(self, x: torch.Tensor) -> kernels.PatternMatrix:
+        if self.binning == 1:
+            return kernels.PatternMatrix(x, None, DTYPE_F32, x.shape[1])
+        b = self.binning
+        pooled = torch.nn.functional.avg_pool2d(x.view(-1, 1, self.net.H, self.net.W), b).view(x.shape[0], -1)
+        out = kernels.PatternMatrix(torch.empty_like(pooled), None, DTYPE_F32, pooled.shape[1])
+        return kernels.prep_patterns(pooled, DTYPE_F32, zero_mean=True, unit_norm=True, out=out)
+
     def local(self, x: torch.Tensor):
         """Proposal + local refinement of prepared patterns x (B, D):
         (orientation (B, 4), cosine (B,), grid index (B,), proposal (B, 4))."""
         q0 = self.net.propose(x)
-        xm = kernels.PatternMatrix(x, None, DTYPE_F32, x.shape[1])
-        val, idx = kernels.local_refine(self.quads, self.rays, xm, q0, self.grid, topk=1)
+        xm = self._local_queries(x)
+        val, idx = kernels.local_refine(self.quads, self.local_rays, xm, q0, self.grid, topk=1)
         j = idx[:, 0].long()
         q = qu_prod(self.grid[j], q0)
         q = q / q.norm(dim=1, keepdim=True)

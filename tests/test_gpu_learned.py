@@ -290,3 +290,21 @@ def test_local_refine_matches_oracle(small_mp, det):
         assert float(ref[i, idx[i, 0]]) >= best - 2e-5
         # sorted descending
         assert bool((val[i, 1:] <= val[i, :-1]).all())
+
+
+def test_binned_local_queries_match_avg_pool():
+    from ebsdtorch_amd.learned import LearnedIndexer, ProposalNet
+    from ebsdtorch_amd.geometry import EBSDGeometry
+    from ebsdtorch_amd.synthetic import band_master_pattern
+
+    geo = EBSDGeometry((32, 32), proj_center=(0.5, 0.5, 0.6))
+    mp = band_master_pattern(41, laue_id=11, n_families=4, seed=1).to(DEV)
+    net = ProposalNet((32, 32), DEV, laue_id=11, compute="fp32", channels=(8, 16, 32, 32), hidden=64)
+    li = LearnedIndexer(net, mp, geo.get_rays().to(DEV), 11, binning=2, local_rays=geo.get_rays((2, 2)).to(DEV))
+    g = torch.Generator().manual_seed(4)
+    x = torch.randn(5, 1024, generator=g)
+    got = li._local_queries(x.to(DEV)).data.cpu().double()
+    ref = torch.nn.functional.avg_pool2d(x.double().view(5, 1, 32, 32), 2).view(5, -1)
+    ref = ref - ref.mean(1, keepdim=True)
+    ref = ref / ref.norm(dim=1, keepdim=True)
+    torch.testing.assert_close(got, ref, atol=1e-6, rtol=0)
