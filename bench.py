@@ -290,7 +290,7 @@ def learned_form(a, mp, dico, dev):
     wall = time.perf_counter() - t0
     rate = n / wall
     # the local-refinement kernel alone (one launch over the batch)
-    xm = kernels.PatternMatrix(x, None, 0, x.shape[1])
+    xm = kernels.PatternMatrix(x.contiguous(), None, 0, x.shape[1])
     q0 = net.propose(x)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()

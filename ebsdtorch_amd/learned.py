@@ -346,10 +346,8 @@ class PatternSimulator:
         noisy = kernels.poisson_gaussian_noise(clean.data, self.n_pix, mc, self.sigma, seed=self.seed,
                                                offset=self.calls)
         self.calls += 1
-        x = kernels.PatternMatrix(torch.empty((n, self.n_pix), dtype=torch.float32, device=dev), None, DTYPE_F32,
-                                  self.n_pix)
-        kernels.prep_patterns(noisy, DTYPE_F32, zero_mean=True, unit_norm=True, out=x)
-        return x.data, q
+        x = kernels.prep_patterns(noisy, DTYPE_F32, zero_mean=True, unit_norm=True)
+        return x.data[:, :self.n_pix], q
 
 
 @dataclass
@@ -457,11 +455,11 @@ class LearnedIndexer:
 
     def _local_queries(self, x: torch.Tensor) -> kernels.PatternMatrix:
         if self.binning == 1:
-            return kernels.PatternMatrix(x, None, DTYPE_F32, x.shape[1])
+            return kernels.PatternMatrix(x.contiguous(), None, DTYPE_F32, x.shape[1])
         b = self.binning
-        pooled = torch.nn.functional.avg_pool2d(x.view(-1, 1, self.net.H, self.net.W), b).view(x.shape[0], -1)
-        out = kernels.PatternMatrix(torch.empty_like(pooled), None, DTYPE_F32, pooled.shape[1])
-        return kernels.prep_patterns(pooled, DTYPE_F32, zero_mean=True, unit_norm=True, out=out)
+        pooled = torch.nn.functional.avg_pool2d(x.contiguous().view(-1, 1, self.net.H, self.net.W), b)
+        pooled = pooled.view(x.shape[0], -1)
+        return kernels.prep_patterns(pooled, DTYPE_F32, zero_mean=True, unit_norm=True)
 
     def local(self, x: torch.Tensor):
         """Proposal + local refinement of prepared patterns x (B, D):
@@ -488,10 +486,8 @@ class LearnedIndexer:
         for s in range(0, n, chunk):
             p = patterns[s:s + chunk].to(dev, torch.float32)
             b = p.shape[0]
-            x = kernels.PatternMatrix(torch.empty((b, p.shape[1]), dtype=torch.float32, device=dev), None,
-                                      DTYPE_F32, p.shape[1])
-            kernels.prep_patterns(p, DTYPE_F32, zero_mean=True, unit_norm=True, out=x)
-            q, v, j, q0 = self.local(x.data)
+            x = kernels.prep_patterns(p, DTYPE_F32, zero_mean=True, unit_norm=True)
+            q, v, j, q0 = self.local(x.data[:, :p.shape[1]])
             need = v < self.threshold
             if self.shell_fallback:
                 need |= self.on_shell[j]
