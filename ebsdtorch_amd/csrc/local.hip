@@ -65,17 +65,40 @@ __device__ __forceinline__ Rot compose_matrix(const float* __restrict__ a, const
   return r;
 }
 
-// Bilinear border lookup (align_corners=False) through the quad table.
-__device__ __forceinline__ float sample_quad(const float4* __restrict__ quads, int H, int W, float gx, float gy) {
-  float ix = fmaf(gx + 1.0f, 0.5f * (float)W, -0.5f);
-  float iy = fmaf(gy + 1.0f, 0.5f * (float)H, -0.5f);
-  ix = fminf(fmaxf(ix, 0.0f), (float)(W - 1));
-  iy = fminf(fmaxf(iy, 0.0f), (float)(H - 1));
+// Side-by-side Lambert map + bilinear border lookup (align_corners=False)
+// through the quad table, for a unit direction v. The same mapping as
+// lambert_sbs + sample_border, with the grid_sample source-index affine maps
+// folded into the Lambert output (ix = u W/4 + W/4 - 1/2 + hemisphere W/2,
+// iy = v H/2 + H/2 - 1/2), clamps as med3 and contraction allowed: the
+// local scores only rank candidates and need no bit-level agreement with
+// the reference's rounding sequence.
+__device__ __forceinline__ float sample_fast(const float4* __restrict__ quads, int H, int W, float qx, float qy,
+                                             float x_hemi, float vx, float vy, float vz) {
+  const float ax = fabsf(vx), ay = fabsf(vy);
+  const float factor = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-2.0f, fabsf(vz), 2.0f), 0.0f));
+  const bool cond = ay <= ax;
+  const float big = cond ? vx : vy;
+  const float sml = cond ? vy : vx;
+  const float abig = cond ? ax : ay;
+  const float sf = big < 0.0f ? -factor : factor;
+  const float ssml = big < 0.0f ? -sml : sml;
+  const float ratio = abig > 0.0f ? ssml * __builtin_amdgcn_rcpf(abig) : 0.0f;
+  const float arct = sf * (atan_unit(ratio) * 0.90031631615710606956f);  // 2 sqrt(2) / pi
+  const float simpler = sf * 0.70710678118654752440f;                    // 2 / sqrt(8)
+  const float u = cond ? simpler : arct;
+  const float v = cond ? arct : simpler;
+  float ix = fmaf(u, qx, (vz < 0.0f) ? x_hemi + 0.5f * (float)W : x_hemi);
+  float iy = fmaf(v, qy, qy - 0.5f);
+  ix = __builtin_amdgcn_fmed3f(ix, 0.0f, (float)(W - 1));
+  iy = __builtin_amdgcn_fmed3f(iy, 0.0f, (float)(H - 1));
   const float x0f = floorf(ix);
   const float y0f = floorf(iy);
   const float w = ix - x0f;
   const float n = iy - y0f;
-  const float4 t = quads[(int)y0f * W + (int)x0f];
+  // texel index y0 W + x0 is exact in float32 (fewer than 2^24 texels, checked
+  // at entry); 32-bit byte offset
+  const uint32_t off = (uint32_t)fmaf(y0f, (float)W, x0f) << 4;
+  const float4 t = *reinterpret_cast<const float4*>(reinterpret_cast<const unsigned char*>(quads) + off);
   const float top = fmaf(w, t.y - t.x, t.x);
   const float bot = fmaf(w, t.w - t.z, t.z);
   return fmaf(n, bot - top, top);
@@ -86,9 +109,8 @@ __device__ __forceinline__ float sample_candidate(const float4* __restrict__ qua
   const float vx = fmaf(R.m[0], rx, fmaf(R.m[1], ry, R.m[2] * rz));
   const float vy = fmaf(R.m[3], rx, fmaf(R.m[4], ry, R.m[5] * rz));
   const float vz = fmaf(R.m[6], rx, fmaf(R.m[7], ry, R.m[8] * rz));
-  float gx, gy;
-  lambert_sbs(vx, vy, vz, gx, gy);
-  return sample_quad(quads, H, W, gx, gy);
+  const float qx = 0.25f * (float)W, qy = 0.5f * (float)H;
+  return sample_fast(quads, H, W, qx, qy, qx - 0.5f, vx, vy, vz);
 }
 
 __global__ __launch_bounds__(256) void quad_table_kernel(const float* __restrict__ mp, int H, int W,
@@ -208,7 +230,7 @@ extern "C" int ebsd_local_refine(const float* quads, int mp_rows, int mp_cols, c
                                  const float* q, int m, int ld, const float* centers, const float* grid,
                                  int n_grid, int topk, float* out_val, int* out_idx, void* stream) {
   if (!quads || !rays || !q || !centers || !grid || !out_val || !out_idx) return EBSD_ERR_NULL;
-  if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols > INT_MAX) return EBSD_ERR_SHAPE;
+  if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols >= (1LL << 24)) return EBSD_ERR_SHAPE;
   if (n_pix < 1 || ld < n_pix || m < 0 || n_grid < 1 || n_grid > kMaxGrid) return EBSD_ERR_SHAPE;
   if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
   if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
