@@ -29,6 +29,8 @@
 #include "common.h"
 #include "lambert.h"
 
+#include <hip/hip_fp16.h>
+
 namespace {
 
 using namespace ebsd_lambert;
@@ -72,7 +74,21 @@ __device__ __forceinline__ Rot compose_matrix(const float* __restrict__ a, const
 // iy = v H/2 + H/2 - 1/2), clamps as med3 and contraction allowed: the
 // local scores only rank candidates and need no bit-level agreement with
 // the reference's rounding sequence.
-__device__ __forceinline__ float sample_fast(const float4* __restrict__ quads, int H, int W, float qx, float qy,
+template <typename QuadT>
+__device__ __forceinline__ float4 load_quad(const QuadT* __restrict__ quads, uint32_t texel);
+template <>
+__device__ __forceinline__ float4 load_quad<float4>(const float4* __restrict__ quads, uint32_t texel) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const unsigned char*>(quads) + (texel << 4));
+}
+template <>
+__device__ __forceinline__ float4 load_quad<ushort4>(const ushort4* __restrict__ quads, uint32_t texel) {
+  const ushort4 h = *reinterpret_cast<const ushort4*>(reinterpret_cast<const unsigned char*>(quads) + (texel << 3));
+  return make_float4(__half2float(__ushort_as_half(h.x)), __half2float(__ushort_as_half(h.y)),
+                     __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
+}
+
+template <typename QuadT>
+__device__ __forceinline__ float sample_fast(const QuadT* __restrict__ quads, int H, int W, float qx, float qy,
                                              float x_hemi, float vx, float vy, float vz) {
   const float ax = fabsf(vx), ay = fabsf(vy);
   const float factor = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-2.0f, fabsf(vz), 2.0f), 0.0f));
@@ -96,15 +112,15 @@ __device__ __forceinline__ float sample_fast(const float4* __restrict__ quads, i
   const float w = ix - x0f;
   const float n = iy - y0f;
   // texel index y0 W + x0 is exact in float32 (fewer than 2^24 texels, checked
-  // at entry); 32-bit byte offset
-  const uint32_t off = (uint32_t)fmaf(y0f, (float)W, x0f) << 4;
-  const float4 t = *reinterpret_cast<const float4*>(reinterpret_cast<const unsigned char*>(quads) + off);
+  // at entry); 32-bit byte offsets
+  const float4 t = load_quad<QuadT>(quads, (uint32_t)fmaf(y0f, (float)W, x0f));
   const float top = fmaf(w, t.y - t.x, t.x);
   const float bot = fmaf(w, t.w - t.z, t.z);
   return fmaf(n, bot - top, top);
 }
 
-__device__ __forceinline__ float sample_candidate(const float4* __restrict__ quads, int H, int W, const Rot& R,
+template <typename QuadT>
+__device__ __forceinline__ float sample_candidate(const QuadT* __restrict__ quads, int H, int W, const Rot& R,
                                                   float rx, float ry, float rz) {
   const float vx = fmaf(R.m[0], rx, fmaf(R.m[1], ry, R.m[2] * rz));
   const float vy = fmaf(R.m[3], rx, fmaf(R.m[4], ry, R.m[5] * rz));
@@ -113,18 +129,30 @@ __device__ __forceinline__ float sample_candidate(const float4* __restrict__ qua
   return sample_fast(quads, H, W, qx, qy, qx - 0.5f, vx, vy, vz);
 }
 
+template <typename QuadT>
+__device__ __forceinline__ QuadT make_quad(float a, float b, float c, float d);
+template <>
+__device__ __forceinline__ float4 make_quad<float4>(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+template <>
+__device__ __forceinline__ ushort4 make_quad<ushort4>(float a, float b, float c, float d) {
+  return make_ushort4(__half_as_ushort(__float2half(a)), __half_as_ushort(__float2half(b)),
+                      __half_as_ushort(__float2half(c)), __half_as_ushort(__float2half(d)));
+}
+
+template <typename QuadT>
 __global__ __launch_bounds__(256) void quad_table_kernel(const float* __restrict__ mp, int H, int W,
-                                                         float4* __restrict__ quads) {
+                                                         QuadT* __restrict__ quads) {
   for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < (int64_t)H * W;
        idx += (int64_t)gridDim.x * 256) {
     const int y = (int)(idx / W), x = (int)(idx % W);
     const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
-    quads[idx] = make_float4(mp[y * W + x], mp[y * W + x1], mp[y1 * W + x], mp[y1 * W + x1]);
+    quads[idx] = make_quad<QuadT>(mp[y * W + x], mp[y * W + x1], mp[y1 * W + x], mp[y1 * W + x1]);
   }
 }
 
+template <typename QuadT>
 __global__ __launch_bounds__(kLocalWaves * 64) void local_refine_kernel(
-    const float4* __restrict__ quads, int H, int W, const float* __restrict__ rays, int n_pix,
+    const QuadT* __restrict__ quads, int H, int W, const float* __restrict__ rays, int n_pix,
     const float* __restrict__ Q, int ld, const float* __restrict__ centers, const float* __restrict__ grid,
     int n_grid, int topk, float* __restrict__ out_val, int* __restrict__ out_idx) {
   __shared__ float scores[kMaxGrid];
@@ -215,19 +243,28 @@ __global__ __launch_bounds__(kLocalWaves * 64) void local_refine_kernel(
 
 }  // namespace
 
-extern "C" int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, float* quads, void* stream) {
+extern "C" int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, void* quads, int dtype,
+                                  void* stream) {
   if (!mp || !quads) return EBSD_ERR_NULL;
   if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols > INT_MAX) return EBSD_ERR_SHAPE;
   if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
   const int64_t n = (int64_t)mp_rows * mp_cols;
-  hipLaunchKernelGGL(quad_table_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), mp, mp_rows, mp_cols, reinterpret_cast<float4*>(quads));
+  const dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == EBSD_DTYPE_F32)
+    hipLaunchKernelGGL(quad_table_kernel<float4>, grid, dim3(256), 0, st, mp, mp_rows, mp_cols,
+                       static_cast<float4*>(quads));
+  else if (dtype == EBSD_DTYPE_F16)
+    hipLaunchKernelGGL(quad_table_kernel<ushort4>, grid, dim3(256), 0, st, mp, mp_rows, mp_cols,
+                       static_cast<ushort4*>(quads));
+  else
+    return EBSD_ERR_DTYPE;
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;
 }
 
-extern "C" int ebsd_local_refine(const float* quads, int mp_rows, int mp_cols, const float* rays, int n_pix,
-                                 const float* q, int m, int ld, const float* centers, const float* grid,
+extern "C" int ebsd_local_refine(const void* quads, int quad_dtype, int mp_rows, int mp_cols, const float* rays,
+                                 int n_pix, const float* q, int m, int ld, const float* centers, const float* grid,
                                  int n_grid, int topk, float* out_val, int* out_idx, void* stream) {
   if (!quads || !rays || !q || !centers || !grid || !out_val || !out_idx) return EBSD_ERR_NULL;
   if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols >= (1LL << 24)) return EBSD_ERR_SHAPE;
@@ -235,9 +272,16 @@ extern "C" int ebsd_local_refine(const float* quads, int mp_rows, int mp_cols, c
   if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
   if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
   if (m == 0) return EBSD_OK;
-  hipLaunchKernelGGL(local_refine_kernel, dim3(m), dim3(kLocalWaves * 64), 0, static_cast<hipStream_t>(stream),
-                     reinterpret_cast<const float4*>(quads), mp_rows, mp_cols, rays, n_pix, q, ld, centers, grid,
-                     n_grid, topk, out_val, out_idx);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g(m), b(kLocalWaves * 64);
+  if (quad_dtype == EBSD_DTYPE_F32)
+    hipLaunchKernelGGL(local_refine_kernel<float4>, g, b, 0, st, static_cast<const float4*>(quads), mp_rows,
+                       mp_cols, rays, n_pix, q, ld, centers, grid, n_grid, topk, out_val, out_idx);
+  else if (quad_dtype == EBSD_DTYPE_F16)
+    hipLaunchKernelGGL(local_refine_kernel<ushort4>, g, b, 0, st, static_cast<const ushort4*>(quads), mp_rows,
+                       mp_cols, rays, n_pix, q, ld, centers, grid, n_grid, topk, out_val, out_idx);
+  else
+    return EBSD_ERR_DTYPE;
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;
 }

@@ -183,9 +183,10 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 //             loads for stage g + 2 are issued as stage g starts, so two
 //             stages are in flight while one is consumed
 //   LDS     : rows are 128 bytes, unpadded; 16-byte chunk c of row r sits at
-//             chunk c ^ (r % 8), so the eight rows read by eight consecutive
-//             lanes hit eight different chunk positions (no bank conflicts)
-//             while each load instruction still writes one contiguous KiB
+//             chunk c ^ ((r / 2) % 8), so the sixteen rows read by sixteen
+//             consecutive lanes hit sixteen different bank groups (see
+//             lp_swz) while each load instruction still writes one
+//             contiguous KiB
 //   top-k   : each lane keeps a list for each of its 2 query columns; after
 //             a dictionary tile every lane offers its 64 fresh scores (FP8:
 //             times the query and dictionary scales, the latter staged in
@@ -206,8 +207,14 @@ constexpr int LP_SLOTS_PER_XCD = 32;
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
+// LDS has 64 four-byte banks: a 128-byte row covers half of them (even rows
+// the lower half, odd rows the upper), and a ds_read_b128 serves 16 lanes at
+// a time. XOR-ing the chunk with (row / 2) % 8 gives the 16 consecutive rows
+// read by 16 lanes 16 different (row parity, chunk) slots: no bank conflict.
+__device__ __forceinline__ int lp_swz_key(int row) { return (row >> 1) & 7; }
+
 __device__ __forceinline__ int lp_swz(int row, int chunk) {
-  return row * STAGE_BYTES + ((chunk ^ (row & 7)) << 4);
+  return row * STAGE_BYTES + ((chunk ^ lp_swz_key(row)) << 4);
 }
 
 template <int DT>
@@ -300,10 +307,12 @@ __global__ __launch_bounds__(LP_THREADS, 1) void sim_topk_lp_kernel(
   const int t_end = min(t_begin + sc.tiles_per_part, n_tiles);
   const int nk = ld_bytes / STAGE_BYTES;
   const int G = max(t_end - t_begin, 0) * nk;  // stages per work unit
-  // direct-load lane mapping: row 8 i + lane / 8 of load i, physical chunk
-  // lane % 8 holds logical chunk (lane % 8) ^ (lane / 8)
+  // direct-load lane mapping: load i writes rows 8 i .. 8 i + 7, lane -> row
+  // 8 i + lane / 8, physical chunk lane % 8, which holds logical chunk
+  // (lane % 8) ^ lp_swz_key(8 i + lane / 8); i = wave + 8 j, so the key
+  // depends on i only through its parity, the wave's parity
   const int lrow = lane >> 3;
-  const int lchunk = (lane & 7) ^ lrow;
+  const int lchunk = (lane & 7) ^ lp_swz_key(8 * (wave & 1) + lrow);
   float* sscale = reinterpret_cast<float*>(smem + LP_SCALE_OFF);
   float* lthr = reinterpret_cast<float*>(smem + LP_THR_OFF);
 

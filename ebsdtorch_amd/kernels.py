@@ -473,14 +473,17 @@ def adam_step(w: torch.Tensor, g: torch.Tensor, m: torch.Tensor, v: torch.Tensor
     check(st, "ebsd_nn_adam")
 
 
-def mp_quad_table(master_pattern: torch.Tensor) -> torch.Tensor:
-    """(S, C) master pattern -> (S, C, 4) float32 bilinear quad table
-    (ebsd_mp_quad_table)."""
+def mp_quad_table(master_pattern: torch.Tensor, dtype: torch.dtype = torch.float32) -> torch.Tensor:
+    """(S, C) master pattern -> (S, C, 4) bilinear quad table in float32 or
+    float16 (ebsd_mp_quad_table)."""
     mp = _f32_on_device(master_pattern, "master_pattern")
     if mp.dim() != 2:
         raise ValueError("master pattern must be a 2D side-by-side image")
-    out = torch.empty((mp.shape[0], mp.shape[1], 4), dtype=torch.float32, device=mp.device)
-    st = _lib.load().ebsd_mp_quad_table(mp.data_ptr(), mp.shape[0], mp.shape[1], out.data_ptr(),
+    dt = {torch.float32: DTYPE_F32, torch.float16: DTYPE_F16}.get(dtype)
+    if dt is None:
+        raise ValueError("the quad table is float32 or float16")
+    out = torch.empty((mp.shape[0], mp.shape[1], 4), dtype=dtype, device=mp.device)
+    st = _lib.load().ebsd_mp_quad_table(mp.data_ptr(), mp.shape[0], mp.shape[1], out.data_ptr(), dt,
                                         _lib.stream_ptr(mp.device))
     check(st, "ebsd_mp_quad_table")
     return out
@@ -490,13 +493,14 @@ def local_refine(quads: torch.Tensor, rays: torch.Tensor, queries: PatternMatrix
                  grid: torch.Tensor, topk: int = 1) -> Tuple[torch.Tensor, torch.Tensor]:
     """Score the local orientations qu_prod(grid[j], centers[i]) of every
     query against it on the fly (ebsd_local_refine). quads: mp_quad_table of
-    the master pattern; queries: FP32 unit-norm pattern rows. Returns
-    (cosine (N, topk) f32, grid index (N, topk) int32)."""
+    the master pattern (float32 or float16); queries: FP32 unit-norm pattern
+    rows. Returns (cosine (N, topk) f32, grid index (N, topk) int32)."""
     if queries.dtype != DTYPE_F32:
         raise ValueError("local refinement scores FP32 queries")
     require_device(quads, "quads")
-    if quads.dtype != torch.float32 or quads.dim() != 3 or quads.shape[2] != 4 or not quads.is_contiguous():
-        raise ValueError("quads must be the (S, C, 4) float32 table of mp_quad_table")
+    qdt = {torch.float32: DTYPE_F32, torch.float16: DTYPE_F16}.get(quads.dtype)
+    if qdt is None or quads.dim() != 3 or quads.shape[2] != 4 or not quads.is_contiguous():
+        raise ValueError("quads must be the (S, C, 4) table of mp_quad_table")
     rays = _f32_on_device(rays, "rays")
     centers = _f32_on_device(centers, "centers")
     grid = _f32_on_device(grid, "grid")
@@ -506,7 +510,7 @@ def local_refine(quads: torch.Tensor, rays: torch.Tensor, queries: PatternMatrix
     dev = centers.device
     val = torch.empty((n, topk), dtype=torch.float32, device=dev)
     idx = torch.empty((n, topk), dtype=torch.int32, device=dev)
-    st = _lib.load().ebsd_local_refine(quads.data_ptr(), quads.shape[0], quads.shape[1], rays.data_ptr(),
+    st = _lib.load().ebsd_local_refine(quads.data_ptr(), qdt, quads.shape[0], quads.shape[1], rays.data_ptr(),
                                        queries.n_pix, queries.data.data_ptr(), n, queries.ld, centers.data_ptr(),
                                        grid.data_ptr(), grid.shape[0], topk, val.data_ptr(), idx.data_ptr(),
                                        _lib.stream_ptr(dev))

@@ -412,7 +412,8 @@ class LearnedIndexer:
     def __init__(self, net: ProposalNet, master_pattern: torch.Tensor, rays: torch.Tensor, laue_id: int,
                  fallback=None, threshold: float = 0.0, grid: Optional[torch.Tensor] = None,
                  radius_deg: float = 3.0, radius_steps: int = 6, shell_fallback: bool = True,
-                 binning: int = 1, local_rays: Optional[torch.Tensor] = None):
+                 binning: int = 1, local_rays: Optional[torch.Tensor] = None,
+                 quad_dtype: torch.dtype = torch.float32):
         self.net = net
         # The local refinement may score virtually binned patterns
         # (average-pooled binning x binning, as ebsd/refining.py's
@@ -449,7 +450,9 @@ class LearnedIndexer:
             shell = torch.zeros(grid.shape[0], dtype=torch.bool)
         self.grid = grid.to(self.device, torch.float32).contiguous()
         self.on_shell = shell.to(self.device)
-        self.quads = kernels.mp_quad_table(self.mp)
+        # float16 taps halve the gather bytes of the local search (relative
+        # error of the master-pattern values <= 2^-11); float32 is exact
+        self.quads = kernels.mp_quad_table(self.mp, quad_dtype)
         if fallback is not None and fallback.n_pix != self.rays.shape[0]:
             raise ValueError("the fallback dictionary must use the same detector pixels")
 

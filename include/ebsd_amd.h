@@ -261,13 +261,14 @@ int ebsd_nn_rot6d_to_quat(const float* z6, int ldz, int b, float* q, void* strea
 
 /*
  * Quad table of a master pattern for ebsd_local_refine: quads (mp_rows,
- * mp_cols, 4) f32, 16-byte aligned, texel (y, x) = the bilinear taps
- * mp[y][x], mp[y][x1], mp[y1][x], mp[y1][x1] with x1 = min(x + 1, cols - 1),
- * y1 = min(y + 1, rows - 1). No counterpart in the reference (a layout of
- * the master pattern that ebsd/master_pattern.py:197-206 samples).
+ * mp_cols, 4) of dtype F32 or F16, 16-byte aligned, texel (y, x) = the
+ * bilinear taps mp[y][x], mp[y][x1], mp[y1][x], mp[y1][x1] with
+ * x1 = min(x + 1, cols - 1), y1 = min(y + 1, rows - 1). No counterpart in the
+ * reference (a layout of the master pattern that ebsd/master_pattern.py:197-206
+ * samples).
  */
-int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, float* quads,
-                       void* stream);
+int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, void* quads,
+                       int dtype, void* stream);
 
 /*
  * Local-dictionary refinement (learned-proposal form): for each query row i
@@ -278,7 +279,8 @@ int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, float* quads,
  * cosine between the zero-mean candidate pattern and the query; the best
  * `topk` per query are written sorted descending (ties: lower j), out_idx
  * holding j (-1 for empty slots). No candidate pattern is stored.
- *   quads (mp_rows, mp_cols, 4) f32 from ebsd_mp_quad_table
+ *   quads (mp_rows, mp_cols, 4) of quad_dtype (F32 or F16) from
+ *   ebsd_mp_quad_table, fewer than 2^24 texels
  *   rays (n_pix, 3) f32; centers (m, 4) f32 proposals
  *   grid (n_grid, 4) f32 local rotations, n_grid <= 4096
  *   out_val (m, topk) f32; out_idx (m, topk) int32
@@ -286,7 +288,7 @@ int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, float* quads,
  * ebsd/refining.py:104-135 (local grid composed with the current orientation,
  * projected and dotted with the experimental pattern).
  */
-int ebsd_local_refine(const float* quads, int mp_rows, int mp_cols,
+int ebsd_local_refine(const void* quads, int quad_dtype, int mp_rows, int mp_cols,
                       const float* rays, int n_pix, const float* q, int m, int ld,
                       const float* centers, const float* grid, int n_grid,
                       int topk, float* out_val, int* out_idx, void* stream);
