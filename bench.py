@@ -116,7 +116,7 @@ def main():
     # time the projection kernel alone on the search dtype
     pe = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     pe[0].record()
-    kernels.project(mp, rays, dico, eng.dtype, out=eng.dictionary)
+    kernels.project(mp, rays, dico, eng.dtype, out=eng.dictionary, quads=eng.quads)
     pe[1].record()
     torch.cuda.synchronize()
     proj_kernel_ms = pe[0].elapsed_time(pe[1])
@@ -240,7 +240,7 @@ def main():
             "flops_per_launch": flops,
         },
         "projection": {
-            "kernel": "ebsd_project", "bound": "hbm", "ms": round(proj_kernel_ms, 3),
+            "kernel": "ebsd_project_quads", "bound": "hbm", "ms": round(proj_kernel_ms, 3),
             "achieved": round(proj_gbps, 1), "peak": PEAKS["hbm_GBps"], "unit": "GB/s",
             "frac": round(proj_gbps / PEAKS["hbm_GBps"], 4), "bytes": proj_bytes,
         },
@@ -373,9 +373,9 @@ def c5_workload(a, dev):
     eng = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True)
     pe = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     pe[0].record()
-    kernels.project(mp, rays, dico, DTYPE_FP8_E4M3, unit_norm=True, out=eng.dictionary)
+    kernels.project(mp, rays, dico, DTYPE_FP8_E4M3, unit_norm=True, out=eng.dictionary, quads=eng.quads)
     pe[1].record()
-    kernels.project(mp, rays, dico, DTYPE_F32, unit_norm=True, out=eng.dictionary_f32)
+    kernels.project(mp, rays, dico, DTYPE_F32, unit_norm=True, out=eng.dictionary_f32, quads=eng.quads)
     pe[2].record()
     torch.cuda.synchronize()
     proj8 = pe[0].elapsed_time(pe[1])

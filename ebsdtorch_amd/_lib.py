@@ -56,6 +56,9 @@ _SIGNATURES = {
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p,
          _c_int, _c_int, _c_void_p, _c_int, _c_void_p],
     ),
+    "ebsd_project_quads": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int,
+                 _c_void_p, _c_int, _c_void_p]),
     "ebsd_project_shifted": (
         _c_int,
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_void_p,

@@ -97,6 +97,29 @@ __device__ __forceinline__ float sample_border(const float* __restrict__ img, in
   return ((v_nw * (s * e) + v_ne * (s * w)) + v_sw * (n * e)) + v_se * (n * w);
 }
 
+// sample_border through a quad table (texel (y, x) holds the taps at (y, x),
+// (y, x1), (y1, x), (y1, x1) with x1, y1 clamped like the scalar path): the
+// same source index, weights, taps and blend order, so the same float32
+// result bit for bit, from one 16-byte load instead of four.
+__device__ __forceinline__ float sample_border_quad(const float4* __restrict__ quads, int H, int W,
+                                                    float gx, float gy) {
+#pragma clang fp contract(off)
+  float ix = (gx + 1.0f) * (0.5f * (float)W) - 0.5f;
+  float iy = (gy + 1.0f) * (0.5f * (float)H) - 0.5f;
+  ix = fminf(fmaxf(ix, 0.0f), (float)(W - 1));
+  iy = fminf(fmaxf(iy, 0.0f), (float)(H - 1));
+  const float x0f = floorf(ix);
+  const float y0f = floorf(iy);
+  const int x0 = (int)x0f;
+  const int y0 = (int)y0f;
+  const float w = ix - x0f;
+  const float e = 1.0f - w;
+  const float n = iy - y0f;
+  const float s = 1.0f - n;
+  const float4 t = quads[y0 * W + x0];
+  return ((t.x * (s * e) + t.y * (s * w)) + t.z * (n * e)) + t.w * (n * w);
+}
+
 // x / |x| with the squared norm accumulated as fma(z, z, fma(y, y, x*x)),
 // the rounding sequence of torch's float32 CPU vector norm over a length-3
 // row (master_pattern.py:166-167).

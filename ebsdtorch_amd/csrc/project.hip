@@ -100,9 +100,10 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
 // patterns; larger detectors re-read them (L1/L2 resident) per pattern.
 template <int ITEMS, typename OutT>
 __global__ __launch_bounds__(256) void project_kernel(
-    const float* __restrict__ mp, int H, int W, const float* __restrict__ rays, int n_pix,
-    const float* __restrict__ quats, int n_ori, OutT* __restrict__ out, int ld,
-    float* __restrict__ out_scale, int flags, const float* __restrict__ offsets) {
+    const float* __restrict__ mp, const float4* __restrict__ quads, int H, int W,
+    const float* __restrict__ rays, int n_pix, const float* __restrict__ quats, int n_ori,
+    OutT* __restrict__ out, int ld, float* __restrict__ out_scale, int flags,
+    const float* __restrict__ offsets) {
   __shared__ float red[4];
   constexpr bool kRaysInRegs = ITEMS <= 16;
   const bool renorm = (flags & EBSD_FLAG_RENORM_DIRS) != 0;
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(256) void project_kernel(
         if (renorm) renormalise(x, y, z);
         float gx, gy;
         lambert_sbs(x, y, z, gx, gy);
-        v = sample_border(mp, H, W, gx, gy);
+        v = quads ? sample_border_quad(quads, H, W, gx, gy) : sample_border(mp, H, W, gx, gy);
       }
       vals[i] = v;
     }
@@ -196,15 +197,15 @@ __global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restric
 template <typename OutT>
 int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, const float* quats,
                    int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st,
-                   const float* offsets = nullptr) {
+                   const float* offsets = nullptr, const float4* quads = nullptr) {
   // ~16 resident workgroups per CU on 256 CUs; each walks n_ori / grid patterns
   const dim3 grid(n_ori < 4096 ? n_ori : 4096), block(256);
   OutT* o = static_cast<OutT*>(out);
   const int items = (n_pix + 255) / 256;
 #define EBSD_PROJ_CASE(I)                                                                        \
   if (items <= I) {                                                                              \
-    hipLaunchKernelGGL((project_kernel<I, OutT>), grid, block, 0, st, mp, H, W, rays, n_pix, quats, \
-                       n_ori, o, ld, out_scale, flags, offsets);                                                 \
+    hipLaunchKernelGGL((project_kernel<I, OutT>), grid, block, 0, st, mp, quads, H, W, rays, n_pix,  \
+                       quats, n_ori, o, ld, out_scale, flags, offsets);                            \
     EBSD_LAUNCH_CHECK();                                                                         \
     return EBSD_OK;                                                                              \
   }
@@ -280,6 +281,36 @@ extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const flo
       return launch_project<__half>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
     case EBSD_DTYPE_FP8_E4M3:
       return launch_project<uint8_t>(mp, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale, flags, st);
+  }
+  return EBSD_ERR_DTYPE;
+}
+
+extern "C" int ebsd_project_quads(const float* quads, int mp_rows, int mp_cols, const float* rays, int n_pix,
+                                  const float* quats, int n_ori, void* out, int out_dtype, int ld_out,
+                                  float* out_scale, int flags, void* stream) {
+  if (!quads || !rays || !quats || !out) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols < 1 || n_ori < 0) return EBSD_ERR_SHAPE;
+  if ((int64_t)mp_rows * mp_cols > (int64_t)INT_MAX) return EBSD_ERR_SHAPE;
+  const int chk = check_ld(out_dtype, n_pix, ld_out);
+  if (chk) return chk;
+  if (out_dtype == EBSD_DTYPE_FP8_E4M3 && !out_scale) return EBSD_ERR_NULL;
+  if (((reinterpret_cast<uintptr_t>(quats) | reinterpret_cast<uintptr_t>(quads)) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (n_ori == 0) return EBSD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float4* q4 = reinterpret_cast<const float4*>(quads);
+  switch (out_dtype) {
+    case EBSD_DTYPE_F32:
+      return launch_project<float>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale,
+                                   flags, st, nullptr, q4);
+    case EBSD_DTYPE_BF16:
+      return launch_project<__hip_bfloat16>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out,
+                                            out_scale, flags, st, nullptr, q4);
+    case EBSD_DTYPE_F16:
+      return launch_project<__half>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale,
+                                    flags, st, nullptr, q4);
+    case EBSD_DTYPE_FP8_E4M3:
+      return launch_project<uint8_t>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale,
+                                     flags, st, nullptr, q4);
   }
   return EBSD_ERR_DTYPE;
 }

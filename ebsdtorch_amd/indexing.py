@@ -86,13 +86,16 @@ class DictionaryIndexer(_ResidentSearch):
         self.unit_norm = bool(dict_unit_norm)
         self.renorm_dirs = bool(renorm_dirs)
         self.n_pix = self.rays.shape[0]
+        # the master pattern's quad table: one 16-byte gather per bilinear
+        # sample in the projection (same values bit for bit)
+        self.quads = kernels.mp_quad_table(self.mp)
         self.dictionary = self._project(self.dtype)
         self.dictionary_f32 = None
         if self.rescore_k > 0 and self.dtype != DTYPE_F32:
             self.dictionary_f32 = self._project(DTYPE_F32)
 
     def _project(self, dtype: int) -> kernels.PatternMatrix:
-        return kernels.project(self.mp, self.rays, self.orientations, dtype, zero_mean=True,
+        return kernels.project(self.mp, self.rays, self.orientations, dtype, zero_mean=True, quads=self.quads,
                                unit_norm=self.unit_norm, renorm_dirs=self.renorm_dirs)
 
     @property
@@ -134,6 +137,7 @@ class PCADictionaryIndexer(_ResidentSearch):
         self.device = learn_orientations.device
         self.mp = master_pattern.to(self.device, torch.float32).contiguous()
         self.coords = coords.to(self.device, torch.float32).contiguous()
+        self.quads = kernels.mp_quad_table(self.mp)
         self.precision = precision
         self.dtype = kernels.PRECISIONS[precision]
         self.rescore_k = int(rescore_k)
@@ -173,7 +177,8 @@ class PCADictionaryIndexer(_ResidentSearch):
             self.dictionary_f32 = dict_f32 if self.rescore_k > 0 else None
 
     def _project(self, quats: torch.Tensor) -> kernels.PatternMatrix:
-        return kernels.project(self.mp, self.coords, quats, DTYPE_F32, zero_mean=True, renorm_dirs=True)
+        return kernels.project(self.mp, self.coords, quats, DTYPE_F32, zero_mean=True, renorm_dirs=True,
+                               quads=self.quads)
 
     @property
     def size(self) -> int:

@@ -89,9 +89,11 @@ def alloc_patterns(n: int, n_pix: int, dtype: int, device: torch.device) -> Patt
 
 def project(master_pattern: torch.Tensor, rays: torch.Tensor, quats: torch.Tensor, dtype: int = DTYPE_F32,
             zero_mean: bool = True, unit_norm: bool = False, renorm_dirs: bool = False,
-            out: Optional[PatternMatrix] = None) -> PatternMatrix:
+            out: Optional[PatternMatrix] = None, quads: Optional[torch.Tensor] = None) -> PatternMatrix:
     """Dictionary projection (ebsd_project). master_pattern (S, 2S) f32,
-    rays (D, 3) f32, quats (N, 4) f32 (w, x, y, z)."""
+    rays (D, 3) f32, quats (N, 4) f32 (w, x, y, z). With `quads`, the
+    master pattern's float32 quad table (mp_quad_table), the gathers read it
+    instead (ebsd_project_quads, same result bit for bit)."""
     for t, name in ((master_pattern, "master_pattern"), (rays, "rays"), (quats, "quats")):
         require_device(t, name)
         if t.dtype != torch.float32:
@@ -113,6 +115,15 @@ def project(master_pattern: torch.Tensor, rays: torch.Tensor, quats: torch.Tenso
     flags = (FLAG_ZERO_MEAN if zero_mean else 0) | (FLAG_UNIT_NORM if unit_norm else 0) | (
         FLAG_RENORM_DIRS if renorm_dirs else 0)
     lib = _lib.load()
+    if quads is not None:
+        require_device(quads, "quads")
+        if quads.dtype != torch.float32 or quads.shape != (mp.shape[0], mp.shape[1], 4) or not quads.is_contiguous():
+            raise ValueError("quads must be the float32 (S, C, 4) quad table of this master pattern")
+        st = lib.ebsd_project_quads(quads.data_ptr(), mp.shape[0], mp.shape[1], rays.data_ptr(), d, quats.data_ptr(),
+                                    n, out.data.data_ptr(), dtype, out.ld, _lib.ptr(out.scale), flags,
+                                    _lib.stream_ptr(quats.device))
+        check(st, "ebsd_project_quads")
+        return out
     st = lib.ebsd_project(mp.data_ptr(), mp.shape[0], mp.shape[1], rays.data_ptr(), d, quats.data_ptr(), n,
                           out.data.data_ptr(), dtype, out.ld, _lib.ptr(out.scale), flags,
                           _lib.stream_ptr(quats.device))

@@ -91,9 +91,10 @@ def simulate_patterns(master_pattern: torch.Tensor, rays: torch.Tensor, orientat
     if mc.dim() == 0:
         mc = mc.expand(n)
     mc = mc.contiguous()
+    quads = kernels.mp_quad_table(master_pattern)
     for c, s in enumerate(range(0, n, chunk)):
         q = orientations[s:s + chunk]
-        clean = kernels.project(master_pattern, rays, q, zero_mean=False)
+        clean = kernels.project(master_pattern, rays, q, zero_mean=False, quads=quads)
         kernels.poisson_gaussian_noise(clean.data, d, mc[s:s + chunk].contiguous(), gaussian_sigma, seed=seed,
                                        offset=c, out=out[s:s + chunk])
     return out

@@ -83,6 +83,18 @@ int ebsd_project(const float* mp, int mp_rows, int mp_cols, const float* rays,
                  void* stream);
 
 /*
+ * ebsd_project reading the master pattern through its F32 quad table
+ * (ebsd_mp_quad_table, dtype F32): one 16-byte load per bilinear sample
+ * instead of four, same taps and blend, so the output equals ebsd_project's
+ * bit for bit. quads (mp_rows, mp_cols, 4) f32, 16-byte aligned.
+ * Replaces: as ebsd_project.
+ */
+int ebsd_project_quads(const float* quads, int mp_rows, int mp_cols,
+                       const float* rays, int n_pix, const float* quats,
+                       int n_ori, void* out, int out_dtype, int ld_out,
+                       float* out_scale, int flags, void* stream);
+
+/*
  * ebsd_project with a per-orientation pattern-centre shift: ray d of row n
  * is normalise(coords[d] + offsets[n]) before the rotation. coords are the
  * un-normalised sample-frame pixel positions, offsets (n_ori, 3) f32 the

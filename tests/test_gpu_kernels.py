@@ -245,3 +245,18 @@ def test_sim_topk_low_precision_streaming_mixes_kernels(dtype):
     for s, c in ((0, 100), (100, 700), (800, 255), (1055, 256), (1311, 1689)):
         kernels.sim_topk(Q, D.narrow(s, c), k, v2, i2, index_base=s, merge=True)
     assert torch.equal(i1, i2) and torch.equal(v1, v2)
+
+
+@pytest.mark.parametrize("dtype", [DTYPE_F32, DTYPE_BF16, DTYPE_FP8_E4M3])
+@pytest.mark.parametrize("shape,renorm", [((60, 60), False), ((33, 47), True)])
+def test_projection_through_quad_table_is_bit_identical(mp101, dtype, shape, renorm):
+    _say(f"projection quad table dtype={dtype} shape={shape}")
+    rays = O.normalized_rays(shape, (0.5, 0.45, 0.6))
+    q = O.sample_ori_fz_laue(11, 500)[:300]
+    mp = mp101.to(DEV)
+    a = kernels.project(mp, rays.to(DEV), q.to(DEV), dtype, renorm_dirs=renorm)
+    quads = kernels.mp_quad_table(mp)
+    b = kernels.project(mp, rays.to(DEV), q.to(DEV), dtype, renorm_dirs=renorm, quads=quads)
+    assert torch.equal(a.data, b.data)
+    if a.scale is not None:
+        assert torch.equal(a.scale, b.scale)
