@@ -403,10 +403,18 @@ extern "C" int ebsd_nn_relu_bwd(float* dy, const float* y, long long n, void* st
   return EBSD_OK;
 }
 
+// Rows per partial sum: 256, or more when that would need more than 65,535
+// partials (the launch's y dimension).
+static int colsum_chunk(int rows) {
+  int chunk = 256;
+  while ((rows + chunk - 1) / chunk > 65535) chunk *= 2;
+  return chunk;
+}
+
 extern "C" int ebsd_nn_colsum_workspace(int rows, int cols, size_t* bytes) {
   if (!bytes) return EBSD_ERR_NULL;
   if (rows < 1 || cols < 1) return EBSD_ERR_SHAPE;
-  const int chunk = 256;
+  const int chunk = colsum_chunk(rows);
   *bytes = (size_t)((rows + chunk - 1) / chunk) * cols * sizeof(float);
   return EBSD_OK;
 }
@@ -415,7 +423,7 @@ extern "C" int ebsd_nn_colsum(const float* x, int rows, int cols, int ld, float*
                               void* stream) {
   if (!x || !out) return EBSD_ERR_NULL;
   if (rows < 1 || cols < 1 || ld < cols) return EBSD_ERR_SHAPE;
-  const int chunk = 256;
+  const int chunk = colsum_chunk(rows);
   const int parts = (rows + chunk - 1) / chunk;
   if (!ws || ws_bytes < (size_t)parts * cols * sizeof(float)) return EBSD_ERR_WORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);

@@ -1,8 +1,9 @@
 // Counter-based noise for on-the-fly pattern simulation (gfx950).
 //
 // Detector noise model of the synthetic workloads: for a clean pattern p
-// (n_pix values) and a per-pattern mean count c,
-//   counts_d ~ Poisson(c * p_d / mean(p)),   out_d = counts_d / c + sigma * N(0, 1).
+// (n_pix values) and a per-pattern mean count c > 0,
+//   counts_d ~ Poisson(c * p_d / mean(p)),   out_d = counts_d / c + sigma * N(0, 1);
+// c <= 0 leaves out Poisson noise: out_d = p_d + sigma * N(0, 1).
 // Random numbers come from Philox4x32-10 (Salmon, Moraes, Dror & Shaw,
 // "Parallel random numbers: as easy as 1, 2, 3", SC'11), keyed by the seed
 // and counted by (pattern, pixel, attempt), so the output is a pure function
@@ -104,20 +105,23 @@ __global__ __launch_bounds__(256) void noise_kernel(const float* __restrict__ cl
     s += vals[i];
   }
   const float mean = block_sum_256(s, red) / (float)n_pix;
+  // a non-positive mean count switches the Poisson stage off (out = clean +
+  // Gaussian); a pattern with non-positive mean gets rate 0
   const float c = mean_counts[n];
-  const float scale = mean > 0.0f ? c / mean : 0.0f;
+  const bool poisson_on = c > 0.0f;
+  const float scale = (poisson_on && mean > 0.0f) ? c / mean : 0.0f;
   float* dst = out + (size_t)n * ld_out;
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const int d = threadIdx.x + i * 256;
     if (d < n_pix) {
       Rng rng{k0, k1, offset, (uint32_t)n, (uint32_t)d, 1u};
-      const int k = poisson(vals[i] * scale, rng);
+      const float signal = poisson_on ? (float)poisson(vals[i] * scale, rng) / c : vals[i];
       // the Gaussian uses its own counter slot (draw 0)
       const U4 g = philox4x32_10(U4{(uint32_t)d, (uint32_t)n, 0u, offset}, k0, k1);
       const float r = sqrtf(-2.0f * logf(u01_open0(g.x)));
       const float z = r * cospif(2.0f * u01(g.y));
-      dst[d] = (float)k / c + sigma * z;
+      dst[d] = signal + sigma * z;
     }
   }
 }

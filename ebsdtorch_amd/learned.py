@@ -117,6 +117,7 @@ class ProposalNet:
         self.adam_v = torch.zeros(off, dtype=torch.float32, device=device)
         self.flat_lp = torch.zeros(off, dtype=torch.bfloat16, device=device) if compute == "bf16" else None
         self.step_count = 0
+        self._acts = self._h1 = None
         self.set_laue(laue_id)
         self._init_weights(seed)
 
@@ -216,8 +217,12 @@ class ProposalNet:
     def backward(self, dz: torch.Tensor):
         """Gradients of the loss w.r.t. every parameter into self.grad, from
         dz = d loss / d z (B, 6) and the activations of forward(keep=True)."""
+        if self._acts is None:
+            raise RuntimeError("backward() needs the activations of a forward(x, keep=True) call")
         acts, h1 = self._acts, self._h1
         B = dz.shape[0]
+        if dz.shape != (acts[0].shape[0], 6) or dz.dtype != torch.float32 or not dz.is_contiguous():
+            raise ValueError("dz must be contiguous float32 (B, 6) for the batch of the kept forward")
         g = self.grad
         # fc2
         kernels.matmul_nt(self._transposed(dz), self._transposed(h1), self.view("fc2.w", g),

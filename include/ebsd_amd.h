@@ -173,7 +173,8 @@ int ebsd_rescore(const float* q, int m, const float* dict, int n, int ld, const 
 /*
  * Detector noise: out[n, d] = Poisson(mean_counts[n] * clean[n, d] /
  * mean_d(clean[n, :])) / mean_counts[n] + sigma * N(0, 1), for the first n_pix
- * columns of each row; Philox4x32-10 counter-based random numbers keyed by
+ * columns of each row (mean_counts[n] <= 0: no Poisson stage, out = clean +
+ * sigma * N(0, 1)); Philox4x32-10 counter-based random numbers keyed by
  * seed, counted by (offset, row, column), so the result does not depend on
  * the launch. No counterpart in the reference (synthetic workload and
  * learned-proposal training data).

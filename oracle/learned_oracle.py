@@ -124,7 +124,8 @@ def poisson_gaussian_noise(clean: np.ndarray, mean_counts: np.ndarray, sigma: fl
                            offset: int = 0) -> Tuple[np.ndarray, np.ndarray]:
     """(counts (n, d) int, out (n, d) float32) of the detector-noise model
     for clean (n, d) float32 patterns, counters (pixel, row, draw, offset),
-    key (seed lo, seed hi); the Gaussian uses draw 0, Poisson draws 1, 2, ..."""
+    key (seed lo, seed hi); the Gaussian uses draw 0, Poisson draws 1, 2, ...
+    A mean count <= 0 leaves out the Poisson stage (out = clean + Gaussian)."""
     clean = np.asarray(clean, dtype=np.float32)
     n, d = clean.shape
     k0, k1 = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
@@ -135,7 +136,8 @@ def poisson_gaussian_noise(clean: np.ndarray, mean_counts: np.ndarray, sigma: fl
         row = clean[i]
         mean = np.float32(row.sum(dtype=np.float64) / d)
         c = np.float32(mean_counts[i])
-        scale = np.float32(c / mean) if mean > 0 else np.float32(0)
+        poisson_on = c > 0
+        scale = np.float32(c / mean) if (poisson_on and mean > 0) else np.float32(0)
         for j in range(d):
             state = {"draw": 1}
 
@@ -144,12 +146,15 @@ def poisson_gaussian_noise(clean: np.ndarray, mean_counts: np.ndarray, sigma: fl
                 state["draw"] += 1
                 return r
 
-            k = _poisson_scalar(np.float32(row[j] * scale), draw)
-            counts[i, j] = k
             g = philox4x32_10(np.array([[j, i, 0, off]], dtype=np.uint32), (k0, k1))[0]
             rr = math.sqrt(-2.0 * math.log(float(_u01_open0(np.uint32(g[0])))))
             z = rr * math.cos(2.0 * math.pi * float(_u01(np.uint32(g[1]))))
-            out[i, j] = np.float32(k / c + sigma * z)
+            if poisson_on:
+                k = _poisson_scalar(np.float32(row[j] * scale), draw)
+                counts[i, j] = k
+                out[i, j] = np.float32(k / c + sigma * z)
+            else:
+                out[i, j] = np.float32(row[j] + sigma * z)
     return counts, out
 
 
@@ -207,7 +212,7 @@ def net_loss_and_grads(params: Dict[str, torch.Tensor], x: torch.Tensor, q_true:
     z = net_forward(p, x.to(dtype), shape)
     loss = geodesic_sym_loss(z, q_true.to(dtype), ops.to(dtype)).mean()
     loss.backward()
-    return float(loss), z.detach(), {k: v.grad.detach() for k, v in p.items()}
+    return float(loss.detach()), z.detach(), {k: v.grad.detach() for k, v in p.items()}
 
 
 def conv_bruteforce(x: torch.Tensor, w: torch.Tensor, b: torch.Tensor) -> torch.Tensor:

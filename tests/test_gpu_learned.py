@@ -43,15 +43,18 @@ def small_mp():
 def test_noise_matches_oracle():
     K = _kernels()
     g = torch.Generator().manual_seed(0)
-    clean = 0.1 + 0.9 * torch.rand(3, 700, generator=g)
-    mc = torch.tensor([3.0, 40.0, 300.0])
-    print("noise 3 x 700", flush=True)
+    clean = 0.1 + 0.9 * torch.rand(4, 700, generator=g)
+    # the last pattern's mean count 0 switches the Poisson stage off
+    mc = torch.tensor([3.0, 40.0, 300.0, 0.0])
+    print("noise 4 x 700", flush=True)
     out = K.poisson_gaussian_noise(clean.to(DEV), 700, mc.to(DEV), 0.05, seed=1234, offset=7).cpu().numpy()
     _, ref = L.poisson_gaussian_noise(clean.numpy(), mc.numpy(), 0.05, 1234, 7)
     match = np.abs(out - ref) < 1e-5
     # float32 rounding of the rate (GPU block sum vs float64 mean) may flip a
     # rare accept/reject decision; everything else is the same random stream
     assert match.mean() > 0.995, match.mean()
+    assert np.isfinite(out).all()
+    np.testing.assert_allclose(out[3], ref[3], atol=1e-5, rtol=0)
 
 
 def test_project_shifted_matches_oracle(small_mp):
@@ -145,6 +148,13 @@ def test_cast_transpose_bias_relu_colsum():
     big = torch.randn(1000, 21, generator=g)
     K.colsum(big.to(DEV), 21, s)
     torch.testing.assert_close(s.cpu().double(), big.double().sum(0), atol=1e-4, rtol=0)
+    # more rows than 65,535 partial sums of 256 rows: wider partials, exact here
+    # (every partial and running sum is a multiple of 64 below 2^25)
+    tall = torch.ones(17_000_000, 1, device=DEV)
+    s1 = torch.empty(1, device=DEV)
+    print("colsum 17M x 1", flush=True)
+    K.colsum(tall, 1, s1)
+    assert float(s1.item()) == 17_000_000.0
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
