@@ -229,7 +229,7 @@ def main():
         "speedup_vs_fp32_di": None if fp32_rate is None else round(value / fp32_rate, 2),
         "roofline": {
             "kernel": "sim_topk (similarity GEMM + fused top-k)" + (
-                " sim_topk_lp_kernel 256x128" if a.precision != "fp32" else " sim_topk_kernel 128x128"),
+                " sim_topk_bt_kernel 192x256" if a.precision != "fp32" else " sim_topk_kernel 128x128"),
             "bound": "mfma",
             "achieved": round(achieved, 2),
             "peak": peak,

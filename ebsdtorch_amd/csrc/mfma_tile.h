@@ -32,23 +32,40 @@ __device__ __forceinline__ int mfma32_row(int r, int lane) {
   return ((r >> 2) << 3) + ((lane >> 5) << 2) + (r & 3);
 }
 
-// One LDS stage of the K loop for a 128 (A) x 32 (B) wave tile.
-// Lane half h = lane / 32 consumes bytes [64 h, 64 h + 64) of each row's
-// 128-byte stage in 16-byte reads; A and B use the same
-// K order, so every MFMA sums matching products whatever the per-instruction
-// K assignment of the hardware.
+// Byte offset of lane half h = lane / 32 inside a row's K stage. Each MFMA
+// of the low-precision types covers one contiguous block of K: FP8 (32x32x64)
+// bytes [64 s, 64 s + 64), lane half h the 32 bytes at 64 s + 32 h; BF16/FP16
+// (32x32x16) bytes [32 s, 32 s + 32), lane half h the 16 bytes at 32 s + 16 h.
+// Every similarity kernel uses this K order, so their FP8/BF16/FP16 scores are
+// the same sums in the same order (bit-identical across kernels). FP32
+// (32x32x2) keeps lane half h on bytes [64 h, 64 h + 64) of the stage.
+template <int DT>
+__device__ __forceinline__ int lane_k_off(int lane) {
+  const int h = lane >> 5;
+  if constexpr (DT == EBSD_DTYPE_F32) return h * 64;
+  else if constexpr (DT == EBSD_DTYPE_FP8_E4M3) return h * 32;
+  else return h * 16;
+}
+
+// Power-of-two part of a positive scale (its exponent field): FP8 row scales
+// act as E8M0 block scales, whose mantissa is implicitly 1.
+__device__ __forceinline__ float pow2_part(float s) { return __uint_as_float(__float_as_uint(s) & 0x7f800000u); }
+
+// One LDS stage of the K loop for a 128 (A) x 32 (B) wave tile; a_off and
+// b_off include lane_k_off. A and B use the same K order, so every MFMA sums
+// matching products.
 template <int DT>
 __device__ __forceinline__ void stage_mfma(const unsigned char* As, const unsigned char* Bs, int a_off,
                                            int b_off, floatx16 (&acc)[4]) {
   if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const intx4 b0 = *reinterpret_cast<const intx4*>(Bs + b_off + s * 32);
-      const intx4 b1 = *reinterpret_cast<const intx4*>(Bs + b_off + s * 32 + 16);
+      const intx4 b0 = *reinterpret_cast<const intx4*>(Bs + b_off + s * 64);
+      const intx4 b1 = *reinterpret_cast<const intx4*>(Bs + b_off + s * 64 + 16);
       const intx8 b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
 #pragma unroll
       for (int bi = 0; bi < 4; ++bi) {
-        const unsigned char* ap = As + a_off + bi * 32 * LDS_ROW + s * 32;
+        const unsigned char* ap = As + a_off + bi * 32 * LDS_ROW + s * 64;
         const intx4 a0 = *reinterpret_cast<const intx4*>(ap);
         const intx4 a1 = *reinterpret_cast<const intx4*>(ap + 16);
         const intx8 a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
@@ -58,13 +75,14 @@ __device__ __forceinline__ void stage_mfma(const unsigned char* As, const unsign
       }
     }
   } else {
+    constexpr int step = DT == EBSD_DTYPE_F32 ? 16 : 32;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const floatx4 braw = *reinterpret_cast<const floatx4*>(Bs + b_off + s * 16);
+      const floatx4 braw = *reinterpret_cast<const floatx4*>(Bs + b_off + s * step);
       floatx4 araw[4];
 #pragma unroll
       for (int bi = 0; bi < 4; ++bi)
-        araw[bi] = *reinterpret_cast<const floatx4*>(As + a_off + bi * 32 * LDS_ROW + s * 16);
+        araw[bi] = *reinterpret_cast<const floatx4*>(As + a_off + bi * 32 * LDS_ROW + s * step);
       if constexpr (DT == EBSD_DTYPE_F32) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)

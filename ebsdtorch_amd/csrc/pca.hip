@@ -113,8 +113,8 @@ __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(const unsigned char* __
     arow[i] = A + (size_t)min(i0 + lr + 32 * i, M - 1) * ld_bytes + lc + (size_t)s0 * STAGE_BYTES;
     brow[i] = B + (size_t)min(j0 + lr + 32 * i, N - 1) * ld_bytes + lc + (size_t)s0 * STAGE_BYTES;
   }
-  const int a_off = (lane & 31) * LDS_ROW + (lane >> 5) * 64;
-  const int b_off = (wave * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 64;
+  const int a_off = (lane & 31) * LDS_ROW + lane_k_off<DT>(lane);
+  const int b_off = (wave * 32 + (lane & 31)) * LDS_ROW + lane_k_off<DT>(lane);
   const int st_off = lr * LDS_ROW + lc;
   floatx16 acc[4];
   tile_product<DT>(arow, brow, ns, smem, a_off, b_off, st_off, acc);

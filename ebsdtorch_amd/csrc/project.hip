@@ -79,9 +79,17 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) am = fmaxf(am, fabsf(vals[i]));
     am = block_max_256(am, red);
-    const float scale = am > 0.0f ? am / kFp8Max : 1.0f;
-    mul = 1.0f / scale;
-    if (tid == 0 && out_scale) out_scale[row] = scale;
+    // the smallest power of two 2^k with am / 2^k <= 448 (the row's largest
+    // magnitude lands in (224, 448]): an E8M0 block scale for the FP8
+    // matrix instructions, and an exact multiplier
+    int k = 0;
+    if (am > 0.0f) {
+      int e;
+      const float m = frexpf(am / kFp8Max, &e);  // am / 448 = m 2^e, m in [0.5, 1)
+      k = min(max(m == 0.5f ? e - 1 : e, -126), 127);
+    }
+    mul = ldexpf(1.0f, -k);
+    if (tid == 0 && out_scale) out_scale[row] = ldexpf(1.0f, k);
   } else {
     if (tid == 0 && out_scale) out_scale[row] = 1.0f;
   }

@@ -6,10 +6,15 @@
 //
 // Element types: FP32 (v_mfma_f32_32x32x2_f32), BF16 / FP16
 // (v_mfma_f32_32x32x16_{bf16,f16}) and OCP FP8 e4m3 (v_mfma_scale_f32_
-// 32x32x64_f8f6f4 with unit block scales; FP8 scores are multiplied by the
-// per-row query and dictionary scales in the epilogue). All accumulate in
-// FP32. Data movement is byte based and identical for every type: a stage is
-// 128 bytes of K per row (32 FP32, 64 BF16/FP16 or 128 FP8 elements).
+// 32x32x64_f8f6f4). FP8 rows carry power-of-two scales: the 192 x 256 kernel
+// feeds their exponents to the MFMA as E8M0 block scales, the 128 x 128 one
+// multiplies its unit-scale scores by the same powers of two, exactly. All
+// accumulate in FP32, in the K order of lane_k_off, so both kernels give the
+// same scores bit for bit.
+//
+// Two kernels: the 128 x 128 one below (every type, any shape), and for
+// BF16 / FP16 / FP8 with at least 192 dictionary rows and 4 stages of K the
+// 192 x 256 one (sim_topk_bt_kernel, further down). The 128 x 128 kernel:
 //
 // Work decomposition
 //   grid    : 512 persistent workgroups = 8 XCDs x 64 resident slots; each
@@ -95,15 +100,15 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 
   // FP8: this lane's query scale (its query column is fixed)
   float qs = 1.0f;
-  if constexpr (DT == EBSD_DTYPE_FP8_E4M3) qs = q_scale[min(q0 + wave * 32 + (lane & 31), M - 1)];
+  if constexpr (DT == EBSD_DTYPE_FP8_E4M3) qs = pow2_part(q_scale[min(q0 + wave * 32 + (lane & 31), M - 1)]);
 
   float tv[K];
   int ti[K];
   topk_init<K>(tv, ti);
 
   const int nk = ld_bytes / STAGE_BYTES;
-  const int a_off = (lane & 31) * LDS_ROW + (lane >> 5) * 64;
-  const int b_off = (wave * 32 + (lane & 31)) * LDS_ROW + (lane >> 5) * 64;
+  const int a_off = (lane & 31) * LDS_ROW + lane_k_off<DT>(lane);
+  const int b_off = (wave * 32 + (lane & 31)) * LDS_ROW + lane_k_off<DT>(lane);
   const int st_off = lr * LDS_ROW + lc;
 
   for (int t = t_begin; t < t_end; ++t) {
@@ -124,7 +129,7 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
         const int d = d0 + bi * 32 + mfma32_row(r, lane);
         if (d < N) {
           float v = acc[bi][r];
-          if constexpr (DT == EBSD_DTYPE_FP8_E4M3) v = v * (qs * d_scale[d]);
+          if constexpr (DT == EBSD_DTYPE_FP8_E4M3) v = v * (qs * pow2_part(d_scale[d]));
           topk_offer<K>(tv, ti, v, index_base + d);
         }
       }
@@ -170,119 +175,261 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Low-precision variant (BF16 / FP16 / FP8): larger tiles, direct-to-LDS
-// loads, a three-stage LDS ring with one barrier per stage.
+// Low-precision variant (BF16 / FP16 / FP8): 192 x 256 block tiles, one
+// 4-wave workgroup per CU, direct-to-LDS loads through a four-stage ring.
 //
-//   block   : 512 threads = 8 waves, one workgroup per CU (256 persistent
-//             workgroups = 8 XCDs x 32); block tile 256 dictionary rows x 128
-//             queries; wave (wd = w % 4, wq = w / 4) owns dictionary rows
-//             [64 wd, 64 wd + 64) and queries [64 wq, 64 wq + 64): 2 x 2 MFMA
-//             32x32 tiles, so every operand byte read from LDS feeds two MFMAs
-//   stages  : 128 bytes of K for all 384 rows (48 KiB) land in LDS straight
-//             from global memory (global_load_lds_dwordx4: 6 per wave); the
-//             loads for stage g + 2 are issued as stage g starts, so two
-//             stages are in flight while one is consumed
-//   LDS     : rows are 128 bytes, unpadded; 16-byte chunk c of row r sits at
-//             chunk c ^ ((r / 2) % 8), so the sixteen rows read by sixteen
-//             consecutive lanes hit sixteen different bank groups (see
-//             lp_swz) while each load instruction still writes one
-//             contiguous KiB
-//   top-k   : each lane keeps a list for each of its 2 query columns; after
-//             a dictionary tile every lane offers its 64 fresh scores (FP8:
-//             times the query and dictionary scales, the latter staged in
-//             LDS per tile); at the end of a work unit the 8 lists of each
-//             query are merged through LDS into one partial list
-constexpr int LP_BM = 256;
-constexpr int LP_BN = 128;
-constexpr int LP_ROWS = LP_BM + LP_BN;
-constexpr int LP_STAGE = LP_ROWS * STAGE_BYTES;   // 49,152 B
-constexpr int LP_NBUF = 3;
-constexpr int LP_SCALE_OFF = LP_NBUF * LP_STAGE;  // two tiles' dictionary scales
-constexpr int LP_THR_OFF = LP_SCALE_OFF + 2 * LP_BM * 4;  // shared list thresholds
-constexpr int LP_LDS = LP_THR_OFF + 2 * 4 * LP_BN * 4;
-constexpr int LP_THREADS = 512;
-constexpr int LP_LOADS = LP_ROWS / 8 / 8;  // 1-KiB direct loads per wave per stage
-static_assert(LP_LOADS == 6, "the stage wait in sim_topk_lp_kernel counts 6 loads per wave");
-constexpr int LP_SLOTS_PER_XCD = 32;
+//   block   : 256 threads = 4 waves, one workgroup per CU (256 persistent
+//             workgroups = 8 XCDs x 32); block tile 192 dictionary rows x 256
+//             queries; wave (wd = w % 2, wq = w / 2) owns dictionary rows
+//             [96 wd, 96 wd + 96) and queries [128 wq, 128 wq + 128): 3 x 4
+//             MFMA 32x32 tiles (192 accumulator registers, leaving the
+//             register allocator a quarter of the AGPR file), so every
+//             operand byte read from LDS feeds three or four MFMAs
+//   stages  : 64 bytes of K for all 448 rows (28 KiB) land in LDS straight
+//             from global memory (global_load_lds_dwordx4, 7 per wave per
+//             stage, plus one 4-byte load of the next tile's dictionary
+//             scales); stage g + 3 is issued as soon as every wave has read
+//             stage g - 1, so three stages are in flight while one is read.
+//             Issuing is branch-free: past the last stage the loads fetch
+//             clamped rows into the free buffer, so every stage waits for the
+//             same count and the MFMAs can be scheduled across the issue
+//   barrier : one per stage, a bare s_barrier: each wave's own vmcnt wait
+//             covers the direct loads it issued, so no full memory fence
+//   LDS     : rows are 64 bytes, unpadded; 16-byte chunk c of row r sits at
+//             chunk c ^ ((r / 4) % 4), so the sixteen rows read by sixteen
+//             consecutive lanes hit sixteen different bank groups (and so do
+//             eight rows of each lane half) while each load instruction still
+//             writes one contiguous KiB
+//   FP8     : the per-row scales are powers of two and enter the MFMA as its
+//             E8M0 block scales (one byte per 32-row block, selected by
+//             op_sel), so scores come out scaled with no epilogue multiply
+//   top-k   : the two lane halves of a 32x32 tile hold the same 32 query
+//             columns but different rows. One v_permlane32_swap per pair of
+//             accumulator registers (column blocks s and s + 2) hands lane
+//             half h the rows of both halves of column block s + 2 h, so each
+//             lane keeps exactly two lists, one per query column it owns.
+//             A tile's scores are screened against the list thresholds; a
+//             32-score block that some lane may take is staged in the wave's
+//             LDS scratch and offered with a uniform index. At the end of a
+//             work unit the 2 lists of each query (one per dictionary wave)
+//             are merged through LDS into one partial list.
+constexpr int BT_D = 192;                                 // dictionary rows per tile
+constexpr int BT_Q = 256;                                 // queries per tile
+constexpr int BT_WD = 96;                                 // dictionary rows per wave
+constexpr int BT_KB = 64;                                 // bytes of K per row per stage
+constexpr int BT_STAGE = (BT_D + BT_Q) * BT_KB;           // 28 KiB
+constexpr int BT_NBUF = 4;
+constexpr int BT_SCALE_OFF = BT_NBUF * BT_STAGE;          // two tiles' dictionary scales
+constexpr int BT_THR_OFF = BT_SCALE_OFF + 2 * 256 * 4;    // thresholds [tile parity][wd][query]
+constexpr int BT_SCR_OFF = BT_THR_OFF + 2 * 2 * BT_Q * 4; // fold scratch, 8 KiB per wave
+constexpr int BT_LDS = BT_SCR_OFF + 4 * 32 * 64 * 4;
+constexpr int BT_THREADS = 256;
+// direct loads per wave per stage: 1-KiB data loads plus one scale load
+constexpr int BT_LOADS = BT_STAGE / 1024 / (BT_THREADS / 64) + 1;
+static_assert(BT_LOADS == 8, "the stage waits of sim_topk_bt_kernel count 8 loads per wave");
+static_assert(BT_LDS <= 160 * 1024, "LDS budget");
+static_assert(BT_Q * 2 * EBSD_MAX_TOPK * 8 <= BT_SCALE_OFF, "the list merge reuses the stage ring");
+constexpr int BT_SLOTS_PER_XCD = 32;
+constexpr int BT_MIN_STAGES = 4;  // the pipeline and the scale slots assume at least 4 stages per tile
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
-// LDS has 64 four-byte banks: a 128-byte row covers half of them (even rows
-// the lower half, odd rows the upper), and a ds_read_b128 serves 16 lanes at
-// a time. XOR-ing the chunk with (row / 2) % 8 gives the 16 consecutive rows
-// read by 16 lanes 16 different (row parity, chunk) slots: no bank conflict.
-__device__ __forceinline__ int lp_swz_key(int row) { return (row >> 1) & 7; }
-
-__device__ __forceinline__ int lp_swz(int row, int chunk) {
-  return row * STAGE_BYTES + ((chunk ^ lp_swz_key(row)) << 4);
+// Direct global -> LDS loads of the 192 x 256 kernel: lane l's 16 (4) bytes
+// land at LDS byte m0 + 16 l (4 l). Written as asm so that the compiler's wait
+// tracking does not see them: it would otherwise wait for every outstanding
+// LDS read before the first MFMA of a stage. Their completion is awaited
+// explicitly (vmcnt in bt_wait_barrier); m0 must be wave-uniform.
+__device__ __forceinline__ void bt_dma16(const void* src, unsigned m0) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(m0)
+               : "memory", "m0");
+}
+__device__ __forceinline__ void bt_dma4(const void* src, unsigned m0) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(src), "s"(m0)
+               : "memory", "m0");
 }
 
+__device__ __forceinline__ int bt_swz(int row, int chunk) { return row * BT_KB + ((chunk ^ ((row >> 2) & 3)) << 4); }
+
+// Wait until at most N of this wave's vector memory operations are in flight
+// (they complete in issue order) and this wave's LDS operations are done,
+// then a bare workgroup barrier. The waits are the builtin (the compiler's
+// wait tracking sees them); the empty asm statements keep memory accesses
+// from moving across.
+template <int N>
+__device__ __forceinline__ void bt_wait_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  // s_waitcnt fields (gfx9): vmcnt[3:0] and [15:14], expcnt[6:4] (7: none),
+  // lgkmcnt[11:8] (0)
+  constexpr int imm = (N & 15) | ((N >> 4) << 14) | (7 << 4);
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(imm);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// MFMA operands of one stage for one wave: A = its 96 dictionary rows (three
+// 32-row blocks), B = its 128 queries (four blocks), 32 bytes of K per lane
+// and block.
+struct BtFrag {
+  intx8 a[3];
+  intx8 b[4];
+};
+
+// Lane half h reads, per 32-row block, the 32 bytes of the 64-byte stage that
+// its MFMAs consume in the K order of lane_k_off: FP8 one 32x32x64 step (bytes
+// 32 h .. 32 h + 31 = chunks 2 h, 2 h + 1), BF16/FP16 two 32x32x16 steps
+// (chunk h, then chunk 2 + h). Reads are issued in first-use order.
 template <int DT>
-__device__ __forceinline__ void lp_stage_mfma(const unsigned char* __restrict__ buf, int wd, int wq, int lane,
-                                              floatx16 (&acc)[2][2]) {
+__device__ __forceinline__ void bt_read(const unsigned char* __restrict__ buf, int wd, int wq, int lane, BtFrag& f) {
   const int r32 = lane & 31, h = lane >> 5;
-  if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+  const int c0 = DT == EBSD_DTYPE_FP8_E4M3 ? 2 * h : h;
+  const int c1 = DT == EBSD_DTYPE_FP8_E4M3 ? 2 * h + 1 : 2 + h;
+  auto rd = [&](int row) {
+    const intx4 x0 = *reinterpret_cast<const intx4*>(buf + bt_swz(row, c0));
+    const intx4 x1 = *reinterpret_cast<const intx4*>(buf + bt_swz(row, c1));
+    return intx8{x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+  };
+  f.a[0] = rd(BT_WD * wd + r32);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) f.b[j] = rd(BT_D + 128 * wq + 32 * j + r32);
+#pragma unroll
+  for (int i = 1; i < 3; ++i) f.a[i] = rd(BT_WD * wd + 32 * i + r32);
+}
+
+// MFMAs IJ .. END - 1 (I = IJ / 4 dictionary block, J = IJ % 4 query block)
+// of K step S of one stage (FP8: one step of 64, BF16/FP16: two of 16).
+// FP8: sa / sb hold the E8M0 exponents of the lane's dictionary / query row
+// in byte i / j.
+template <int DT, int S, int IJ, int END>
+__device__ __forceinline__ void bt_mfma_step(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4]) {
+  if constexpr (IJ < END) {
+    constexpr int I = IJ / 4, J = IJ % 4;
+    if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+      acc[I][J] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(f.a[I], f.b[J], acc[I][J], 0, 0, I, sa, J, sb);
+    } else {
+      const intx4 a4 = {f.a[I][4 * S], f.a[I][4 * S + 1], f.a[I][4 * S + 2], f.a[I][4 * S + 3]};
+      const intx4 b4 = {f.b[J][4 * S], f.b[J][4 * S + 1], f.b[J][4 * S + 2], f.b[J][4 * S + 3]};
+      if constexpr (DT == EBSD_DTYPE_BF16)
+        acc[I][J] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a4),
+                                                            __builtin_bit_cast(bf16x8, b4), acc[I][J], 0, 0, 0);
+      else
+        acc[I][J] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(halfx8, a4),
+                                                           __builtin_bit_cast(halfx8, b4), acc[I][J], 0, 0, 0);
+    }
+    bt_mfma_step<DT, S, IJ + 1, END>(f, sa, sb, acc);
+  }
+}
+
+// All MFMAs of one stage, with hook(k) called after the k-th of the first 8
+// (k = 0 .. 7): the loads of a later stage go out in the shadow of the MFMAs.
+template <int DT, int IJ = 0, typename Hook>
+__device__ __forceinline__ void bt_mfma_hooked(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4], Hook&& hook) {
+  if constexpr (IJ < 12) {
+    bt_mfma_step<DT, 0, IJ, IJ + 1>(f, sa, sb, acc);
+    if constexpr (IJ < 8) hook(IJ);
+    bt_mfma_hooked<DT, IJ + 1>(f, sa, sb, acc, hook);
+  } else if constexpr (DT != EBSD_DTYPE_FP8_E4M3) {
+    bt_mfma_step<DT, 1, 0, 12>(f, sa, sb, acc);
+  }
+}
+
+// MFMAs of dictionary blocks [I0, I1) of one stage (both K steps for
+// BF16/FP16, step-major so consecutive MFMAs are independent).
+template <int DT, int I0, int I1>
+__device__ __forceinline__ void bt_mfma_rows(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4]) {
+  bt_mfma_step<DT, 0, 4 * I0, 4 * I1>(f, sa, sb, acc);
+  if constexpr (DT != EBSD_DTYPE_FP8_E4M3) bt_mfma_step<DT, 1, 4 * I0, 4 * I1>(f, sa, sb, acc);
+}
+
+__device__ __forceinline__ int e8m0_of(float s) { return (int)((__float_as_uint(s) >> 23) & 0xffu); }
+
+// Reads an accumulator register where it is used (the asm operand pins the
+// value to its AGPR), so the compiler does not copy all accumulators to VGPRs
+// ahead of the fold.
+__device__ __forceinline__ float bt_acc(float a) {
+  float v;
+  asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(v) : "a"(a));
+  return v;
+}
+
+// Zeroes an accumulator register in place (the asm result is pinned to an
+// AGPR, so the accumulators' home stays the AGPR file).
+__device__ __forceinline__ void bt_zero(floatx16& a) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    float z;
+    asm volatile("v_accvgpr_write_b32 %0, 0" : "=a"(z));
+    a[r] = z;
+  }
+}
+
+// v_permlane32_swap of two accumulator registers: x keeps lanes 0-31 of `lo`
+// and gets lanes 0-31 of `hi` in lanes 32-63; y gets lanes 32-63 of `lo` in
+// lanes 0-31 and keeps lanes 32-63 of `hi`.
+__device__ __forceinline__ void bt_swap(float lo, float hi, float& x, float& y) {
+  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(bt_acc(lo)), __float_as_uint(bt_acc(hi)), false,
+                                                  false);
+  x = __uint_as_float(p[0]);
+  y = __uint_as_float(p[1]);
+}
+
+// Fold a finished 192 x 256 tile into the lane's two lists. dbase: index of
+// the wave's first dictionary row; n_valid: how many of its 96 rows exist;
+// thr[s]: a lower bound of the final K-th best score of list s's query;
+// scratch: the wave's 8 KiB of LDS, element e of lane l at float 64 e + l.
+template <int K>
+__device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)[2][K], int (&ti)[2][K],
+                                        const float (&thr)[2], int dbase, int n_valid, float* scratch, int lane) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const int c0 = 4 * s + 2 * h;
-      intx8 a[2], b[2];
+      // element e = 2 r + x: accumulator register r of lane half x, i.e. row
+      // 8 (r / 4) + 4 x + r % 4 of the 32-row block i, for column block
+      // s + 2 h (h = this lane's half)
+      float mx = -INFINITY;
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int ra = 64 * wd + 32 * i + r32;
-        const int rb = LP_BM + 64 * wq + 32 * i + r32;
-        const intx4 a0 = *reinterpret_cast<const intx4*>(buf + lp_swz(ra, c0));
-        const intx4 a1 = *reinterpret_cast<const intx4*>(buf + lp_swz(ra, c0 + 1));
-        const intx4 b0 = *reinterpret_cast<const intx4*>(buf + lp_swz(rb, c0));
-        const intx4 b1 = *reinterpret_cast<const intx4*>(buf + lp_swz(rb, c0 + 1));
-        a[i] = intx8{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
-        b[i] = intx8{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+      for (int r = 0; r < 16; ++r) {
+        float x, y;
+        bt_swap(acc[i][s][r], acc[i][s + 2][r], x, y);
+        mx = fmaxf(mx, fmaxf(x, y));
       }
+      if (__builtin_amdgcn_ballot_w64(!(mx < thr[s])) == 0) continue;
+      // some lane may take a score: stage the block's 32 scores in LDS and
+      // walk them with a uniform index (one offer site per block)
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0, kUnitScale,
-                                                                      0, kUnitScale);
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int c = 4 * h + s;
-      floatx4 a[2], b[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        a[i] = *reinterpret_cast<const floatx4*>(buf + lp_swz(64 * wd + 32 * i + r32, c));
-        b[i] = *reinterpret_cast<const floatx4*>(buf + lp_swz(LP_BM + 64 * wq + 32 * i + r32, c));
+      for (int r = 0; r < 16; ++r) {
+        float x, y;
+        bt_swap(acc[i][s][r], acc[i][s + 2][r], x, y);
+        scratch[64 * (2 * r) + lane] = x;
+        scratch[64 * (2 * r + 1) + lane] = y;
       }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          if constexpr (DT == EBSD_DTYPE_BF16)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[i]),
-                                                                __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
-          else
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(halfx8, a[i]),
-                                                               __builtin_bit_cast(halfx8, b[j]), acc[i][j], 0, 0, 0);
-        }
+      float vn = scratch[lane];
+#pragma unroll 1
+      for (int e = 0; e < 32; ++e) {
+        const float v = vn;
+        vn = scratch[64 * min(e + 1, 31) + lane];
+        const int r = e >> 1;
+        const int row = 32 * i + ((r >> 2) << 3) + ((e & 1) << 2) + (r & 3);
+        if (row < n_valid && !(v < thr[s])) topk_offer<K>(tv[s], ti[s], v, dbase + row);
+      }
     }
   }
 }
 
-// Requires N >= 256 and at least 3 stages (ld_bytes >= 384); see plan_sim.
-// Wave 0 issues 4 extra loads (the scales) with the first stage of each tile,
-// so its wait for "at most one batch outstanding" may cover a few loads of
-// the next batch too: conservative, never short.
+// Requires N >= BT_D and at least BT_MIN_STAGES stages (ld_bytes >= 256); see
+// plan_sim. FP8 scales are used through their exponents (powers of two).
 template <int DT, int K>
-__global__ __launch_bounds__(LP_THREADS, 1) void sim_topk_lp_kernel(
+__global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     const unsigned char* __restrict__ Q, const float* __restrict__ q_scale, int M,
     const unsigned char* __restrict__ Dm, const float* __restrict__ d_scale, int N, int ld_bytes,
     SimSched sc, float* __restrict__ pval, int* __restrict__ pidx, int index_base) {
-  __shared__ __attribute__((aligned(16))) unsigned char smem[LP_LDS];
+  __shared__ __attribute__((aligned(16))) unsigned char smem[BT_LDS];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wd = wave & 3, wq = wave >> 2;
+  const int wd = wave & 1, wq = wave >> 1;
+  const int r32 = lane & 31, h = lane >> 5;
   // Workgroups are dispatched round-robin over the XCDs: blockIdx % 8 is the
   // XCD, blockIdx / 8 the slot (a, b): query tile a of the XCD's group of A,
   // dictionary part b (see the FP32 kernel for the grouping).
@@ -302,187 +449,185 @@ __global__ __launch_bounds__(LP_THREADS, 1) void sim_topk_lp_kernel(
     if (super >= sc.R) return;
   }
   const int part = super * sc.B + b;
-  const int n_tiles = (N + LP_BM - 1) / LP_BM;
+  const int n_tiles = (N + BT_D - 1) / BT_D;
   const int t_begin = part * sc.tiles_per_part;
   const int t_end = min(t_begin + sc.tiles_per_part, n_tiles);
-  const int nk = ld_bytes / STAGE_BYTES;
+  const int nk = ld_bytes / BT_KB;
   const int G = max(t_end - t_begin, 0) * nk;  // stages per work unit
-  // direct-load lane mapping: load i writes rows 8 i .. 8 i + 7, lane -> row
-  // 8 i + lane / 8, physical chunk lane % 8, which holds logical chunk
-  // (lane % 8) ^ lp_swz_key(8 i + lane / 8); i = wave + 8 j, so the key
-  // depends on i only through its parity, the wave's parity
-  const int lrow = lane >> 3;
-  const int lchunk = (lane & 7) ^ lp_swz_key(8 * (wave & 1) + lrow);
-  float* sscale = reinterpret_cast<float*>(smem + LP_SCALE_OFF);
-  float* lthr = reinterpret_cast<float*>(smem + LP_THR_OFF);
+  // direct-load lane mapping: load L writes rows 16 L .. 16 L + 15 of the
+  // stage (rows < 192 dictionary, then queries), lane -> row 16 L + lane / 4
+  // at physical chunk lane % 4, which holds logical chunk
+  // (lane % 4) ^ ((row / 4) % 4) = (lane % 4) ^ ((lane / 16) % 4); wave w
+  // issues loads L = w + 4 j, j < 7: three dictionary blocks, four query blocks
+  const int lrow = lane >> 2;
+  const int lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  const float* sscale = reinterpret_cast<const float*>(smem + BT_SCALE_OFF);
+  float* lthr = reinterpret_cast<float*>(smem + BT_THR_OFF);
+  float* scratch = reinterpret_cast<float*>(smem + BT_SCR_OFF) + wave * 32 * 64;
 
   for (int grp = g_first; grp < sc.n_qgroups; grp += g_step) {
     const int qt = grp * sc.A + a;
     if (qt >= sc.n_qtiles) continue;  // uniform over the workgroup
     // (a part with no tiles, G == 0, still writes its empty lists)
-    const int q0 = qt * LP_BN;
-
-    float qs[2] = {1.0f, 1.0f};
+    const int q0 = qt * BT_Q;
+    // this wave's four query loads of every stage: query rows
+    // 16 (wave + 4 j) + lrow, j < 4 (loads L = 12 + wave + 4 j)
+    const unsigned char* qsrc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      qsrc[j] = Q + (size_t)min(q0 + 16 * (wave + 4 * j) + lrow, M - 1) * ld_bytes + lchunk * 16;
+    // FP8: E8M0 exponents of the lane's query rows, byte j for block j
+    int sb = 0;
     if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
 #pragma unroll
-      for (int j = 0; j < 2; ++j) qs[j] = q_scale[min(q0 + 64 * wq + 32 * j + (lane & 31), M - 1)];
+      for (int j = 0; j < 4; ++j) sb |= e8m0_of(q_scale[min(q0 + 128 * wq + 32 * j + r32, M - 1)]) << (8 * j);
     }
     float tv[2][K];
     int ti[2][K];
     topk_init<K>(tv[0], ti[0]);
     topk_init<K>(tv[1], ti[1]);
-    floatx16 acc[2][2];
+    floatx16 acc[3][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+      for (int j = 0; j < 4; ++j) bt_zero(acc[i][j]);
 
-
-    auto issue = [&](int g) {
-      const int t = t_begin + g / nk;
-      const int kt = g - (g / nk) * nk;
-      unsigned char* dst = smem + (g % LP_NBUF) * LP_STAGE;
-      const int d0 = t * LP_BM;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int i = wave + 8 * j;
-        const unsigned char* src = Dm + (size_t)min(d0 + 8 * i + lrow, N - 1) * ld_bytes + lchunk * 16 +
-                                   kt * STAGE_BYTES;
-        __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(dst + i * 1024), 16, 0, 0);
-      }
-#pragma unroll
-      for (int j = 4; j < 6; ++j) {
-        const int i = wave + 8 * j;  // query rows (loads 0-3 cover dictionary rows)
-        const unsigned char* src = Q + (size_t)min(q0 + 8 * i + lrow - LP_BM, M - 1) * ld_bytes + lchunk * 16 +
-                                   kt * STAGE_BYTES;
-        __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(dst + i * 1024), 16, 0, 0);
-      }
-      if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
-        // the tile's 256 dictionary scales, 4-byte direct loads (clamped rows)
-        if (kt == 0 && wave == 0) {
-          unsigned char* sdst = smem + LP_SCALE_OFF + (t & 1) * LP_BM * 4;
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            __builtin_amdgcn_global_load_lds(d_scale + min(d0 + 64 * e + lane, N - 1), (lds_ptr_t)(sdst + e * 256),
-                                             4, 0, 0);
-        }
-      }
+    // Loads of stage g, in 8 parts: k < 3 dictionary rows 16 (wave + 4 k) ..,
+    // k = 3 .. 6 query rows 16 (wave + 4 (k - 3)) .., k = 7 the dictionary
+    // scales of the tile after g's (tile t's into LDS slot t % 2: 256
+    // entries, 64 per wave, rows clamped, entries 192 .. 255 unused; BF16 /
+    // FP16 load bytes of the dictionary instead, so every batch has 8 loads).
+    // All addresses are clamped, also for g >= G; the buffer is g % BT_NBUF.
+    const unsigned lds0 = (unsigned)(uintptr_t)(lds_ptr_t)smem;
+    const unsigned wave_s = __builtin_amdgcn_readfirstlane(wave);
+    auto scale_src = [&](int t) -> const void* {
+      if constexpr (DT == EBSD_DTYPE_FP8_E4M3)
+        return d_scale + min(t * BT_D + 64 * wave + lane, N - 1);
+      else
+        return Dm + (size_t)min(t * BT_D, N - 1) * ld_bytes + 4 * lane;
     };
-
-    if (G > 0) issue(0);
-    if (G > 1) issue(1);
-    for (int g = 0; g < G; ++g) {
+    auto scale_m0 = [&](int t) { return lds0 + BT_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
+    auto issue_part = [&](int g, int k) {
       const int tl = g / nk;
       const int kt = g - tl * nk;
-      // this wave's loads of stage g are complete once at most the next
-      // stage's batch is outstanding
-      if (g + 1 < G)
-        asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
-      else
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      // stage g (and, at a tile's first stage, the tile's dictionary scales,
-      // issued right after it) visible to all waves; every wave is done with
-      // stage g - 1, whose buffer the loads of stage g + 2 overwrite, and
-      // with the epilogue of the tile two back, whose scale slot the next
-      // tile's scales overwrite
-      __syncthreads();
-      if (g + 2 < G) issue(g + 2);
-      lp_stage_mfma<DT>(smem + (g % LP_NBUF) * LP_STAGE, wd, wq, lane, acc);
-
-      if (kt == nk - 1) {
-        const int d0 = (t_begin + tl) * LP_BM;
-        const float* tsc = sscale + ((t_begin + tl) & 1) * LP_BM;
-        // A score below the 16th best of any of the query's 8 lists cannot
-        // reach its final top-K (the union's K-th best is at least as high),
-        // so offers are filtered with the largest list threshold known: the
-        // two lane halves of this wave now, the other three dictionary waves
-        // as of the previous tile (thresholds only rise).
-        float thr[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const float own = fmaxf(tv[j][K - 1], __shfl_xor(tv[j][K - 1], 32, 64));
-          const int ql = 64 * wq + 32 * j + (lane & 31);
-          float t = own;
-          if (tl > 0) {
-            const float* prev = lthr + ((tl - 1) & 1) * 4 * LP_BN;
-#pragma unroll
-            for (int w = 0; w < 4; ++w) t = fmaxf(t, prev[w * LP_BN + ql]);
-          }
-          thr[j] = t;
-          if (lane < 32) lthr[(tl & 1) * 4 * LP_BN + wd * LP_BN + ql] = own;
-        }
-        // One offer site per (list, tile row block), looping over the 16
-        // accumulator rows with a uniform register index: the epilogue stays
-        // a few hundred instructions (fully unrolled, its 64 inlined list
-        // insertions would not fit the instruction cache).
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int i = 0; i < 2; ++i) {
-#pragma unroll 1
-            for (int r = 0; r < 16; ++r) {
-              const int dl = 64 * wd + 32 * i + mfma32_row(r, lane);
-              float v = acc[i][j][r];
-              if (d0 + dl < N) {
-                if constexpr (DT == EBSD_DTYPE_FP8_E4M3) v = v * (qs[j] * tsc[dl]);
-                if (!(v < thr[j])) topk_offer<K>(tv[j], ti[j], v, index_base + d0 + dl);
-              }
-            }
-          }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+      const int t = t_begin + tl;
+      const unsigned mb = lds0 + (g % BT_NBUF) * BT_STAGE;
+      if (k < 3) {
+        const int L = wave + 4 * k;
+        const unsigned char* src =
+            Dm + (size_t)min(t * BT_D + 16 * L + lrow, N - 1) * ld_bytes + lchunk * 16 + kt * BT_KB;
+        bt_dma16(src, mb + (wave_s + 4 * k) * 1024);
+      } else if (k < 7) {
+        bt_dma16(qsrc[k - 3] + kt * BT_KB, mb + (12 + wave_s + 4 * (k - 3)) * 1024);
+      } else {
+        bt_dma4(scale_src(t + 1), scale_m0(t + 1));
       }
-    }
+    };
+    auto issue = [&](int g) {
+#pragma unroll
+      for (int k = 0; k < BT_LOADS; ++k) issue_part(g, k);
+    };
+    // E8M0 exponents of the lane's dictionary rows of tile t (byte i: block
+    // i). The result passes through a move where it is read, so its LDS wait
+    // is settled here and not at the MFMAs of the next stage.
+    auto dict_scales = [&](int t) {
+      int s = 0;
+      if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+        const float* sl = sscale + (t & 1) * 256 + BT_WD * wd + r32;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s |= e8m0_of(sl[32 * i]) << (8 * i);
+        asm volatile("v_mov_b32 %0, %1" : "=v"(s) : "v"(s));
+      }
+      return s;
+    };
 
-    // Merge the 8 lists of each query: [128 queries][8 lists][K] in LDS.
+    // Stage g is complete once the two batches issued after it are the only
+    // ones in flight (loads complete in issue order); the barrier then also
+    // means every wave is done reading the stages before g.
+    // prologue: the first tile's scales and stages 0 .. 2 in flight
+    int sa = 0;
+    if (G > 0) {
+      bt_dma4(scale_src(t_begin), scale_m0(t_begin));
+      for (int p = 0; p < BT_NBUF - 1; ++p) issue(p);
+      bt_wait_barrier<2 * BT_LOADS>();
+      sa = dict_scales(t_begin);
+      __builtin_amdgcn_s_waitcnt(0xC07F);
+    }
+    const int n_t = G / max(nk, 1);
+    for (int tl = 0; tl < n_t; ++tl) {
+      for (int kt = 0; kt < nk; ++kt) {
+        const int g = tl * nk + kt;
+        bt_wait_barrier<2 * BT_LOADS>();
+        BtFrag f;
+        bt_read<DT>(smem + (g % BT_NBUF) * BT_STAGE, wd, wq, lane, f);
+        __builtin_amdgcn_sched_barrier(0);
+        // the MFMAs start as their operands come in; the loads of stage
+        // g + 3 (into the buffer of stage g - 1) go out between them
+        bt_mfma_hooked<DT>(f, sa, sb, acc, [&](int k) { issue_part(g + BT_NBUF - 1, k); });
+      }
+      // A score below the K-th best of either of the query's 2 lists cannot
+      // reach its final top-K, so offers are filtered with the larger list
+      // threshold known: this wave's now, the other dictionary wave's as of
+      // the previous tile (thresholds only rise).
+      float thr[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int ql = 128 * wq + 32 * (s + 2 * h) + r32;
+        const float own = tv[s][K - 1];
+        float th = own;
+        if (tl > 0) th = fmaxf(th, lthr[((tl - 1) & 1) * 2 * BT_Q + (wd ^ 1) * BT_Q + ql]);
+        thr[s] = th;
+        lthr[(tl & 1) * 2 * BT_Q + wd * BT_Q + ql] = own;
+      }
+      const int d0w = (t_begin + tl) * BT_D + BT_WD * wd;
+      bt_fold<K>(acc, tv, ti, thr, index_base + d0w, N - d0w, scratch, lane);
+      // the next tile's scales came with this tile's stages
+      if (tl + 1 < n_t) sa = dict_scales(t_begin + tl + 1);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bt_zero(acc[i][j]);
+      // no LDS operation of the fold stays in flight into the next tile's
+      // stages (lgkmcnt(0); vmcnt and expcnt unconstrained), so the waits
+      // there only count the stage's own reads
+      __builtin_amdgcn_s_waitcnt(0xC07F);
+    }
+    // the clamped loads past the last stage land before the LDS is reused
+    bt_wait_barrier<0>();
+
+    // Merge the 2 lists of each query: [256 queries][2 lists][K] in LDS (no
+    // direct load is in flight: the last stage waited for all of them).
     __syncthreads();
     float* cv = reinterpret_cast<float*>(smem);
-    int* ci = reinterpret_cast<int*>(smem) + LP_BN * 8 * K;
+    int* ci = reinterpret_cast<int*>(smem) + BT_Q * 2 * K;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int ql = 64 * wq + 32 * j + (lane & 31);
-      const int list = wd * 2 + (lane >> 5);
+    for (int s = 0; s < 2; ++s) {
+      const int ql = 128 * wq + 32 * (s + 2 * h) + r32;
 #pragma unroll
       for (int e = 0; e < K; ++e) {
-        cv[(ql * 8 + list) * K + e] = tv[j][e];
-        ci[(ql * 8 + list) * K + e] = ti[j][e];
+        cv[(ql * 2 + wd) * K + e] = tv[s][e];
+        ci[(ql * 2 + wd) * K + e] = ti[s][e];
       }
     }
     __syncthreads();
-    if (tid < LP_BN) {
+    {
       const int q = q0 + tid;
-      const float* v = cv + tid * 8 * K;
-      const int* ix = ci + tid * 8 * K;
-      int head[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const float* v = cv + tid * 2 * K;
+      const int* ix = ci + tid * 2 * K;
+      int h0 = 0, h1 = 0;
       for (int e = 0; e < K; ++e) {
-        int best = -1;
-        float bv = -INFINITY;
-        int bi = INT_MAX;
-#pragma unroll
-        for (int l = 0; l < 8; ++l) {
-          if (head[l] < K) {
-            const float hv = v[l * K + head[l]];
-            const int hi = ix[l * K + head[l]];
-            if (best < 0 || topk_better(hv, hi, bv, bi)) {
-              best = l;
-              bv = hv;
-              bi = hi;
-            }
-          }
-        }
-#pragma unroll
-        for (int l = 0; l < 8; ++l) head[l] += (l == best) ? 1 : 0;
+        // a list can only run dry on the K-th pick, so both heads stay < K
+        // whenever they are read
+        const float v0 = v[h0], v1 = v[K + h1];
+        const int i0 = ix[h0], i1 = ix[K + h1];
+        const bool take1 = topk_better(v1, i1, v0, i0);
         if (q < M) {
           const size_t o = (size_t)(part * K + e) * M + q;
-          pval[o] = bv;
-          pidx[o] = bi;
+          pval[o] = take1 ? v1 : v0;
+          pidx[o] = take1 ? i1 : i0;
         }
+        h0 += take1 ? 0 : 1;
+        h1 += take1 ? 1 : 0;
       }
     }
     // the lists are read before the next unit's loads overwrite the LDS
@@ -564,17 +709,21 @@ __global__ __launch_bounds__(256) void rescore_kernel(
 
 struct SimPlan {
   int k_list;   // register list length (1 or 16)
-  bool lp;      // low-precision kernel (sim_topk_lp_kernel) instead of the 128 x 128 one
+  bool bt;      // 192 x 256 low-precision kernel (sim_topk_bt_kernel) instead of the 128 x 128 one
   SimSched sc;
   int n_parts;  // partial lists per query (= B * R)
   int grid;
 };
 
 // 8 XCDs x 32 CUs on MI355X. The 128 x 128 kernel runs two resident
-// workgroups per CU (64 slots per XCD), the low-precision one a single
-// 8-wave workgroup per CU (32 slots).
+// workgroups per CU (64 slots per XCD), the 192 x 256 one a single 4-wave
+// workgroup per CU (32 slots).
 constexpr int kXcds = 8;
 constexpr int kSlotsPerXcd = 64;
+// Query tiles per XCD group of the 192 x 256 kernel (A of plan_groups): 16
+// tiles x 2 dictionary parts; each dictionary tile is then streamed by 16 CUs
+// of an XCD at once, which measured fastest (A = 2, 4, 8 were 5-20% slower).
+constexpr int kBtGroupTiles = 16;
 
 void plan_groups(SimSched& s, int slots, int a_max, int n_tiles, int& n_parts) {
   // Largest A <= a_max that still gives every XCD a query group.
@@ -591,11 +740,11 @@ SimPlan plan_sim(int m, int n, int topk, int dtype, int ld_bytes) {
   SimPlan p;
   p.k_list = topk <= 1 ? 1 : EBSD_MAX_TOPK;
   SimSched& s = p.sc;
-  p.lp = dtype != EBSD_DTYPE_F32 && n >= LP_BM && ld_bytes / STAGE_BYTES >= 3;
-  if (p.lp) {
-    s.n_qtiles = (m + LP_BN - 1) / LP_BN;
-    plan_groups(s, LP_SLOTS_PER_XCD, 16, (n + LP_BM - 1) / LP_BM, p.n_parts);
-    p.grid = kXcds * LP_SLOTS_PER_XCD;
+  p.bt = dtype != EBSD_DTYPE_F32 && n >= BT_D && ld_bytes / BT_KB >= BT_MIN_STAGES;
+  if (p.bt) {
+    s.n_qtiles = (m + BT_Q - 1) / BT_Q;
+    plan_groups(s, BT_SLOTS_PER_XCD, kBtGroupTiles, (n + BT_D - 1) / BT_D, p.n_parts);
+    p.grid = kXcds * BT_SLOTS_PER_XCD;
   } else {
     s.n_qtiles = (m + TILE - 1) / TILE;
     plan_groups(s, kSlotsPerXcd, 8, (n + TILE - 1) / TILE, p.n_parts);
@@ -641,7 +790,7 @@ extern "C" int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t
   if (elem_bytes(dtype) == 0) return EBSD_ERR_DTYPE;
   // the plan also depends on the row length; cover both kernels
   const size_t b1 = workspace_bytes(plan_sim(m, n, topk, EBSD_DTYPE_F32, STAGE_BYTES), m);
-  const size_t b2 = workspace_bytes(plan_sim(m, n, topk, dtype, 3 * STAGE_BYTES), m);
+  const size_t b2 = workspace_bytes(plan_sim(m, n, topk, dtype, BT_MIN_STAGES * BT_KB), m);
   *bytes = b1 > b2 ? b1 : b2;
   return EBSD_OK;
 }
@@ -651,8 +800,8 @@ static void launch_sim_kernel(const SimPlan& p, const unsigned char* Qb, const f
                               const unsigned char* Db, const float* d_scale, int n, int ld_bytes, float* pval,
                               int* pidx, int index_base, hipStream_t st) {
   if constexpr (DT != EBSD_DTYPE_F32) {
-    if (p.lp) {
-      hipLaunchKernelGGL((sim_topk_lp_kernel<DT, K>), dim3(p.grid), dim3(LP_THREADS), 0, st, Qb, q_scale, m, Db,
+    if (p.bt) {
+      hipLaunchKernelGGL((sim_topk_bt_kernel<DT, K>), dim3(p.grid), dim3(BT_THREADS), 0, st, Qb, q_scale, m, Db,
                          d_scale, n, ld_bytes, p.sc, pval, pidx, index_base);
       return;
     }

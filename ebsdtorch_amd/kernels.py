@@ -48,7 +48,8 @@ class PatternMatrix:
     """Rows of zero-padded patterns in one dtype, as the kernels consume them.
 
     data  (n, ld) tensor of TORCH_DTYPE[dtype] (uint8 bytes for FP8)
-    scale (n,) float32 per-row scale (FP8: value = stored * scale), or None
+    scale (n,) float32 per-row scale (FP8: value = stored * scale, a power of
+          two), or None
     """
 
     __slots__ = ("data", "scale", "dtype", "n_pix")

@@ -43,7 +43,7 @@ extern "C" {
 #define EBSD_DTYPE_F32 0
 #define EBSD_DTYPE_BF16 1
 #define EBSD_DTYPE_F16 2
-#define EBSD_DTYPE_FP8_E4M3 3 /* OCP e4m3fn, per-row scale in a float array */
+#define EBSD_DTYPE_FP8_E4M3 3 /* OCP e4m3fn, per-row power-of-two scale in a float array */
 
 /* flags of ebsd_project / ebsd_prep_patterns */
 #define EBSD_FLAG_ZERO_MEAN 1   /* subtract the per-pattern mean            */
@@ -71,7 +71,9 @@ int ebsd_pad_multiple(int dtype);
  *   rays     (n_pix, 3) f32 sample-frame directions
  *   quats    (n_ori, 4) f32 unit quaternions (w, x, y, z)
  *   out      (n_ori, ld_out) of out_dtype; out_scale (n_ori,) f32 is written
- *            for FP8 output (value = stored * scale) and may be NULL otherwise
+ *            for FP8 output (value = stored * scale, scale = the smallest
+ *            power of two with max|value| / scale <= 448) and may be NULL
+ *            otherwise
  * Replaces: ebsd/indexing.py:117-143 (qu_apply + MasterPattern.interpolate +
  * mean subtraction), s2_and_so3/quaternions.py:186-213 (qu_apply),
  * s2_and_so3/sphere.py:145-185 (rosca_lambert_side_by_side),
@@ -139,7 +141,9 @@ int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t* bytes);
 
 /*
  * Similarity + streaming top-k: scores S = Q * Dict^T (angular metric, FP32
- * accumulation; FP8 scores are multiplied by q_scale[i] * d_scale[j]) are
+ * accumulation; FP8 scores are multiplied by q_scale[i] * d_scale[j], each
+ * scale taken as its power of two: only the exponent is used, as the E8M0
+ * block scale of the FP8 matrix instructions) are
  * reduced on the fly to the best `topk` dictionary rows per query; the score
  * matrix is never written. Indices are index_base + row. With merge != 0 the
  * existing (top_val, top_idx) state, sorted descending, is merged with the

@@ -77,10 +77,14 @@ def test_projection_low_precision_outputs(mp101, dtype):
     lp = kernels.project(mp101.to(DEV), rays.to(DEV), q.to(DEV), dtype)
     assert lp.ld % kernels.pad_multiple(dtype) == 0
     if dtype == DTYPE_FP8_E4M3:
-        # stored bytes are the OCP e4m3 rounding of value / scale
-        scale = f32.abs().amax(1) / 448.0
-        assert torch.allclose(lp.scale, scale, rtol=1e-6)
-        expect = (f32 * (1.0 / scale)[:, None]).to(torch.float8_e4m3fn).view(torch.uint8)
+        # the scale is the smallest power of two 2^k with amax / 2^k <= 448,
+        # and the stored bytes are the OCP e4m3 rounding of value / 2^k
+        m, e = torch.frexp(f32.abs().amax(1) / 448.0)
+        k = torch.where(m == 0.5, e - 1, e)
+        assert torch.equal(lp.scale, torch.ldexp(torch.ones_like(m), k))
+        amax = f32.abs().amax(1)
+        assert torch.all(amax / lp.scale <= 448) and torch.all(amax / lp.scale > 224)
+        expect = torch.ldexp(f32, -k[:, None]).to(torch.float8_e4m3fn).view(torch.uint8)
         assert torch.equal(lp.data[:, :3600], expect)
         assert torch.all(lp.data[:, 3600:] == 0)
     else:
@@ -208,12 +212,14 @@ def test_sim_topk_low_precision_exact_on_stored_values(dtype, m, n, d, k):
     _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
 
 
-@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_FP8_E4M3])
+@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_F16, DTYPE_FP8_E4M3])
 @pytest.mark.parametrize("m,n,d,k", [(5, 256, 384, 16), (1000, 5000, 400, 16), (777, 300, 1000, 1),
-                                     (130, 70000, 520, 8), (4096, 2048, 3600, 16)])
+                                     (130, 70000, 520, 8), (4096, 2048, 3600, 16), (300, 192, 256, 16),
+                                     (257, 1000, 128, 4), (2000, 20000, 640, 16)])
 def test_sim_topk_large_tile_kernel(dtype, m, n, d, k):
-    """Shapes that take the 256 x 128 low-precision kernel (N >= 256, at
-    least three 128-byte K stages): ragged M and N, one and many parts."""
+    """Shapes that take the 192 x 256 low-precision kernel (N >= 192, at
+    least four 64-byte K stages): ragged M and N, a single tile, the minimum
+    stage count (FP8 d=256, BF16/FP16 d=128), one and many parts."""
     _say(f"sim_topk large-tile dtype={dtype} m={m} n={n} d={d} k={k}")
     g = torch.Generator().manual_seed(m + n + d + k + dtype)
     q = torch.randn(m, d, generator=g)
@@ -226,10 +232,10 @@ def test_sim_topk_large_tile_kernel(dtype, m, n, d, k):
     _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
 
 
-@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_FP8_E4M3])
+@pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_F16, DTYPE_FP8_E4M3])
 def test_sim_topk_low_precision_streaming_mixes_kernels(dtype):
-    """Chunks below 256 rows take the 128 x 128 kernel, larger ones the
-    256 x 128 kernel; the streamed result equals one pass bit for bit."""
+    """Chunks below 192 rows take the 128 x 128 kernel, larger ones the
+    192 x 256 kernel; the streamed result equals one pass bit for bit."""
     _say(f"sim_topk low-precision streaming dtype={dtype}")
     g = torch.Generator().manual_seed(17 + dtype)
     q = torch.randn(300, 512, generator=g)
@@ -242,7 +248,7 @@ def test_sim_topk_low_precision_streaming_mixes_kernels(dtype):
     kernels.sim_topk(Q, D, k, v1, i1, merge=False)
     v2 = torch.full((300, k), float("-inf"), device=DEV)
     i2 = torch.full((300, k), -1, dtype=torch.int32, device=DEV)
-    for s, c in ((0, 100), (100, 700), (800, 255), (1055, 256), (1311, 1689)):
+    for s, c in ((0, 100), (100, 700), (800, 150), (950, 191), (1141, 192), (1333, 1667)):
         kernels.sim_topk(Q, D.narrow(s, c), k, v2, i2, index_base=s, merge=True)
     assert torch.equal(i1, i2) and torch.equal(v1, v2)
 
