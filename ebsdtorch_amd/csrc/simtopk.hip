@@ -395,23 +395,26 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)
         mx = fmaxf(mx, fmaxf(x, y));
       }
       if (__builtin_amdgcn_ballot_w64(!(mx < thr[s])) == 0) continue;
-      // some lane may take a score: stage the block's 32 scores in LDS and
-      // walk them with a uniform index (one offer site per block)
+      // Some lane may take a score: stage the block's 32 scores in LDS, mark
+      // the ones that pass the threshold, and let every lane walk only its
+      // own marked scores (a divergent loop of max-popcount trips, not 32).
+      unsigned m = 0;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         float x, y;
         bt_swap(acc[i][s][r], acc[i][s + 2][r], x, y);
         scratch[64 * (2 * r) + lane] = x;
         scratch[64 * (2 * r + 1) + lane] = y;
+        m |= (!(x < thr[s]) ? 1u : 0u) << (2 * r);
+        m |= (!(y < thr[s]) ? 1u : 0u) << (2 * r + 1);
       }
-      float vn = scratch[lane];
-#pragma unroll 1
-      for (int e = 0; e < 32; ++e) {
-        const float v = vn;
-        vn = scratch[64 * min(e + 1, 31) + lane];
+      while (m != 0) {
+        const int e = __builtin_ctz(m);
+        m &= m - 1;
+        const float v = scratch[64 * e + lane];
         const int r = e >> 1;
         const int row = 32 * i + ((r >> 2) << 3) + ((e & 1) << 2) + (r & 3);
-        if (row < n_valid && !(v < thr[s])) topk_offer<K>(tv[s], ti[s], v, dbase + row);
+        if (row < n_valid) topk_offer<K>(tv[s], ti[s], v, dbase + row);
       }
     }
   }
