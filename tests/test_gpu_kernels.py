@@ -266,3 +266,30 @@ def test_projection_through_quad_table_is_bit_identical(mp101, dtype, shape, ren
     assert torch.equal(a.data, b.data)
     if a.scale is not None:
         assert torch.equal(a.scale, b.scale)
+
+
+@pytest.mark.parametrize("n", [150, 2000])
+def test_sim_topk_fp8_uses_the_exponent_of_row_scales(n):
+    """FP8 row scales act as E8M0 block scales: only their power of two
+    counts, in both similarity kernels (N < 192: 128 x 128, else 192 x 256).
+    Scaling every scale by 1.5 (same exponent) must not change a bit of the
+    result; doubling them doubles the scores exactly."""
+    _say(f"sim_topk fp8 scale exponent n={n}")
+    g = torch.Generator().manual_seed(21 + n)
+    q = torch.randn(300, 520, generator=g)
+    dd = torch.randn(n, 520, generator=g)
+    Q = kernels.prep_patterns(q.to(DEV), DTYPE_FP8_E4M3, zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), DTYPE_FP8_E4M3, zero_mean=False)
+    assert torch.equal(torch.frexp(D.scale)[0], torch.full_like(D.scale, 0.5))  # powers of two
+    k = 8
+    v1 = torch.empty(300, k, device=DEV)
+    i1 = torch.empty(300, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, v1, i1, merge=False)
+    D15 = type(D)(D.data, D.scale * 1.5, D.dtype, D.n_pix)
+    v2 = torch.empty_like(v1)
+    i2 = torch.empty_like(i1)
+    kernels.sim_topk(Q, D15, k, v2, i2, merge=False)
+    assert torch.equal(v1, v2) and torch.equal(i1, i2)
+    D2 = type(D)(D.data, D.scale * 2.0, D.dtype, D.n_pix)
+    kernels.sim_topk(Q, D2, k, v2, i2, merge=False)
+    assert torch.equal(v2, 2.0 * v1) and torch.equal(i1, i2)
