@@ -255,9 +255,10 @@ def learned_form(a, mp, dico, dev):
     on `learned_train` patterns simulated on the fly (uniform orientations,
     Poisson mean 100-400 + N(0, 0.05^2), pattern centre jittered +-0.01),
     then `learned_batch` noisy patterns at seeded orientations indexed by
-    proposal -> 2,197-orientation local dictionary within 3 degrees -> FP8
-    full-dictionary search (FP32 top-16 rescore) below the calibrated
-    threshold. Compared with plain FP32 DI of the same 1M dictionary on a
+    proposal -> 2,197-orientation local dictionary within 3 degrees, searched
+    on 4x4-binned patterns (16x16), its best 8 rescored on the full 64x64
+    patterns -> FP8 full-dictionary search (FP32 top-16 rescore) below the
+    calibrated threshold. Compared with plain FP32 DI of the same 1M dictionary on a
     sample, and with the ground truth (disorientation)."""
     import torch
 
@@ -276,7 +277,9 @@ def learned_form(a, mp, dico, dev):
     log(f"learned: trained on {tlog.patterns} patterns in {tlog.seconds:.1f}s "
         f"({tlog.patterns / tlog.seconds:.0f}/s), loss {tlog.loss_curve[0][1]:.3f} -> {tlog.loss_curve[-1][1]:.4f}")
     fb = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True)
-    idx = LearnedIndexer(net, mp, rays, 11, fallback=fb, threshold=0.0)
+    binning, rescore_k = 4, 8
+    idx = LearnedIndexer(net, mp, rays, 11, fallback=fb, threshold=0.0, binning=binning,
+                         local_rays=geom.get_rays((binning, binning)).to(dev), rescore_k=rescore_k)
     cal = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=6)
     idx.threshold = calibrate_threshold(idx, cal, n=8192, max_error_deg=1.0, precision=0.999)
     test = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=7)
@@ -289,16 +292,22 @@ def learned_form(a, mp, dico, dev):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     rate = n / wall
-    # the local-refinement kernel alone (one launch over the batch)
-    xm = kernels.PatternMatrix(x.contiguous(), None, 0, x.shape[1])
+    # the local search (binned) and the full-resolution rescore, one launch
+    # each over the batch
+    xb = idx._local_queries(x)
+    xf = kernels.PatternMatrix(x.contiguous(), None, 0, x.shape[1])
     q0 = net.propose(x)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     e0.record()
-    kernels.local_refine(idx.quads, rays, xm, q0, idx.grid, topk=1)
+    _, cand = kernels.local_refine(idx.quads, idx.local_rays, xb, q0, idx.grid, topk=rescore_k)
     e1.record()
+    kernels.local_rescore(idx.quads, rays, xf, q0, idx.grid, cand)
+    e2.record()
     torch.cuda.synchronize()
     local_ms = e0.elapsed_time(e1)
-    samples = n * idx.grid.shape[0] * x.shape[1]
+    rescore_ms = e1.elapsed_time(e2)
+    samples = n * idx.grid.shape[0] * xb.n_pix
+    rescore_samples = n * rescore_k * x.shape[1]
     err = torch.rad2deg(disori_angle_laue(res["orientations"].double(), q_true.double(), 11))
     qs = torch.tensor([0.5, 0.9, 0.99, 0.999], dtype=torch.float64, device=err.device)
     quant = [round(v, 3) for v in torch.quantile(err, qs).tolist()]
@@ -318,7 +327,8 @@ def learned_form(a, mp, dico, dev):
     del ref, fb
     torch.cuda.empty_cache()
     out = {
-        "workload": f"learned proposal: {n} patterns 64x64, local dictionary {idx.grid.shape[0]} within 3 deg, "
+        "workload": f"learned proposal: {n} patterns 64x64, local dictionary {idx.grid.shape[0]} within 3 deg "
+                    f"on {binning}x{binning}-binned patterns, best {rescore_k} rescored at 64x64, "
                     f"fallback FP8 DI over {dico.shape[0]}",
         "patterns_per_s": round(rate, 1),
         "fp32_di_patterns_per_s": round(fp32_rate, 1),
@@ -335,9 +345,11 @@ def learned_form(a, mp, dico, dev):
                   "compute": "bf16 operands, fp32 master weights"},
         "local_refine_kernel": {"ms": round(local_ms, 3), "samples": samples,
                                 "gsamples_per_s": round(samples / (local_ms / 1000) / 1e9, 1)},
+        "local_rescore_kernel": {"ms": round(rescore_ms, 3), "samples": rescore_samples,
+                                 "gsamples_per_s": round(rescore_samples / (rescore_ms / 1000) / 1e9, 1)},
     }
     log(f"learned: {rate:.0f} patterns/s vs FP32 DI {fp32_rate:.0f}/s, fallback {fallback_rate:.4f}, "
-        f"err quantiles {quant}, local kernel {local_ms:.1f} ms")
+        f"err quantiles {quant}, local kernel {local_ms:.1f} ms, rescore kernel {rescore_ms:.1f} ms")
     return out
 
 
@@ -388,10 +400,12 @@ def c5_workload(a, dev):
     net = ProposalNet((det, det), dev, laue_id=laue, compute="bf16", seed=0)
     tlog = train_proposal_net(net, sim, a.c5_train, batch=256, lr=1e-3, warmup=200, log_every=1000)
     log(f"c5: trained on {tlog.patterns} patterns in {tlog.seconds:.1f}s, loss {tlog.loss_curve[-1][1]:.4f}")
-    # the local refinement scores 2x2-binned patterns (60x60) against the
-    # 2,197 candidates; the network and the fallback search see 120x120
-    lidx = LearnedIndexer(net, mp, rays, laue, fallback=eng, threshold=0.0, binning=2,
-                          local_rays=geom.get_rays((2, 2)).to(dev))
+    # the local search scores 4x4-binned patterns (30x30) against the 2,197
+    # candidates and rescores its best 8 at 120x120; the network and the
+    # fallback search see 120x120
+    c5_binning = 4
+    lidx = LearnedIndexer(net, mp, rays, laue, fallback=eng, threshold=0.0, binning=c5_binning,
+                          local_rays=geom.get_rays((c5_binning, c5_binning)).to(dev), rescore_k=8)
     lidx.threshold = calibrate_threshold(
         lidx, PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=21),
         n=8192)
@@ -457,7 +471,7 @@ def c5_workload(a, dev):
                        "fp32_ms": round(proj32, 2), "fp32_GBps": round(bytes32 / proj32 / 1e6, 1),
                        "hbm_peak_GBps": PEAKS["hbm_GBps"]},
         "learned_proposal": {"patterns_per_s": round(a.c5_scan / t_le, 1), "fallback_rate": round(float(fb.float().mean()), 4),
-                             "local_binning": 2,
+                             "local_binning": c5_binning, "rescore_k": lidx.rescore_k,
                              "threshold": round(lidx.threshold, 5),
                              "disorientation_deg_quantiles_vs_truth": dict(zip(["p50", "p90", "p99", "p99.9"],
                                                                                [round(v, 3) for v in torch.quantile(err_le, qs).tolist()])),

@@ -241,6 +241,87 @@ __global__ __launch_bounds__(kLocalWaves * 64) void local_refine_kernel(
   }
 }
 
+// Rescoring of k listed candidates per query (k <= 64), typically the top-k
+// of a local search on binned patterns, against the full-resolution query.
+// One wave per query: lane = slice * kp + c with kp the power of two >= k,
+// so candidate c (lanes c >= k idle) is scored by 64 / kp lanes, slice s
+// taking pixels s, s + 64/kp, ...; a pixel's query value and ray are then
+// shared by the kp lanes of a slice and consecutive slices read consecutive
+// pixels. The sums (about the candidate's first sample, as in the search)
+// are combined over the slices by butterflies, and the best candidate of the
+// query by a butterfly over c (higher score, then lower list position).
+template <typename QuadT>
+__global__ __launch_bounds__(kLocalWaves * 64) void local_rescore_kernel(
+    const QuadT* __restrict__ quads, int H, int W, const float* __restrict__ rays, int n_pix,
+    const float* __restrict__ Q, int ld, int m, const float* __restrict__ centers, const float* __restrict__ grid,
+    int n_grid, const int* __restrict__ cand, int k, int kp, float* __restrict__ out_val,
+    float* __restrict__ best_val, int* __restrict__ best_idx) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kLocalWaves + (threadIdx.x >> 6);
+  if (i >= m) return;  // whole waves only; the kernel has no block barrier
+  const int c = lane & (kp - 1);
+  const int n_slices = 64 / kp;
+  const int slice = lane / kp;
+  const int j = cand[(size_t)i * k + min(c, k - 1)];
+  const bool valid = c < k && j >= 0 && j < n_grid;
+  const float* __restrict__ xrow = Q + (size_t)i * ld;
+
+  const Rot R = compose_matrix(grid + (size_t)(valid ? j : 0) * 4, centers + (size_t)i * 4);
+  const float p0 = sample_candidate(quads, H, W, R, rays[0], rays[1], rays[2]);
+  float sx[2] = {0.0f, 0.0f}, sp[2] = {0.0f, 0.0f}, spp[2] = {0.0f, 0.0f}, sxp[2] = {0.0f, 0.0f};
+  int d = slice;
+  for (; d + n_slices < n_pix; d += 2 * n_slices) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = d + u * n_slices;
+      const float* r = rays + 3 * e;
+      const float xv = xrow[e];
+      const float p = sample_candidate(quads, H, W, R, r[0], r[1], r[2]) - p0;
+      sx[u] += xv;
+      sp[u] += p;
+      spp[u] = fmaf(p, p, spp[u]);
+      sxp[u] = fmaf(xv, p, sxp[u]);
+    }
+  }
+  if (d < n_pix) {
+    const float* r = rays + 3 * d;
+    const float xv = xrow[d];
+    const float p = sample_candidate(quads, H, W, R, r[0], r[1], r[2]) - p0;
+    sx[0] += xv;
+    sp[0] += p;
+    spp[0] = fmaf(p, p, spp[0]);
+    sxp[0] = fmaf(xv, p, sxp[0]);
+  }
+  float Sx = sx[0] + sx[1], Sp = sp[0] + sp[1], Spp = spp[0] + spp[1], Sxp = sxp[0] + sxp[1];
+  for (int off = kp; off < 64; off <<= 1) {
+    Sx += __shfl_xor(Sx, off, 64);
+    Sp += __shfl_xor(Sp, off, 64);
+    Spp += __shfl_xor(Spp, off, 64);
+    Sxp += __shfl_xor(Sxp, off, 64);
+  }
+  const float inv_d = 1.0f / (float)n_pix;
+  const float var = Spp - Sp * Sp * inv_d;
+  const float num = Sxp - Sp * Sx * inv_d;
+  const float score = valid ? (var > 0.0f ? num * rsqrtf(var) : 0.0f) : -INFINITY;
+  if (out_val && slice == 0 && c < k) out_val[(size_t)i * k + c] = score;
+
+  float bv = score;
+  int bc = c < k ? c : INT_MAX;
+  for (int off = 1; off < kp; off <<= 1) {
+    const float ov = __shfl_xor(bv, off, 64);
+    const int oc = __shfl_xor(bc, off, 64);
+    if (topk_better(ov, oc, bv, bc)) {
+      bv = ov;
+      bc = oc;
+    }
+  }
+  if (lane == 0) {
+    const bool found = bc != INT_MAX && bv != -INFINITY;
+    best_val[i] = found ? bv : -INFINITY;
+    best_idx[i] = found ? cand[(size_t)i * k + bc] : -1;
+  }
+}
+
 }  // namespace
 
 extern "C" int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, void* quads, int dtype,
@@ -280,6 +361,34 @@ extern "C" int ebsd_local_refine(const void* quads, int quad_dtype, int mp_rows,
   else if (quad_dtype == EBSD_DTYPE_F16)
     hipLaunchKernelGGL(local_refine_kernel<ushort4>, g, b, 0, st, static_cast<const ushort4*>(quads), mp_rows,
                        mp_cols, rays, n_pix, q, ld, centers, grid, n_grid, topk, out_val, out_idx);
+  else
+    return EBSD_ERR_DTYPE;
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_local_rescore(const void* quads, int quad_dtype, int mp_rows, int mp_cols, const float* rays,
+                                  int n_pix, const float* q, int m, int ld, const float* centers, const float* grid,
+                                  int n_grid, const int* cand, int k, float* out_val, float* best_val,
+                                  int* best_idx, void* stream) {
+  if (!quads || !rays || !q || !centers || !grid || !cand || !best_val || !best_idx) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols < 1 || (long long)mp_rows * mp_cols >= (1LL << 24)) return EBSD_ERR_SHAPE;
+  if (n_pix < 1 || ld < n_pix || m < 0 || n_grid < 1) return EBSD_ERR_SHAPE;
+  if (k < 1 || k > 64) return EBSD_ERR_TOPK;
+  if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (m == 0) return EBSD_OK;
+  int kp = 1;
+  while (kp < k) kp <<= 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g((unsigned)((m + kLocalWaves - 1) / kLocalWaves)), b(kLocalWaves * 64);
+  if (quad_dtype == EBSD_DTYPE_F32)
+    hipLaunchKernelGGL(local_rescore_kernel<float4>, g, b, 0, st, static_cast<const float4*>(quads), mp_rows,
+                       mp_cols, rays, n_pix, q, ld, m, centers, grid, n_grid, cand, k, kp, out_val, best_val,
+                       best_idx);
+  else if (quad_dtype == EBSD_DTYPE_F16)
+    hipLaunchKernelGGL(local_rescore_kernel<ushort4>, g, b, 0, st, static_cast<const ushort4*>(quads), mp_rows,
+                       mp_cols, rays, n_pix, q, ld, m, centers, grid, n_grid, cand, k, kp, out_val, best_val,
+                       best_idx);
   else
     return EBSD_ERR_DTYPE;
   EBSD_LAUNCH_CHECK();

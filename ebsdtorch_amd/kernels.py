@@ -530,6 +530,42 @@ def local_refine(quads: torch.Tensor, rays: torch.Tensor, queries: PatternMatrix
     return val, idx
 
 
+def local_rescore(quads: torch.Tensor, rays: torch.Tensor, queries: PatternMatrix, centers: torch.Tensor,
+                  grid: torch.Tensor, cand: torch.Tensor, all_scores: bool = False):
+    """Rescore listed candidates qu_prod(grid[cand[i, c]], centers[i]) of every
+    query at the queries' own resolution (ebsd_local_rescore). cand: (N, k)
+    grid indices, k <= 64, -1 = none. Returns (best cosine (N,) f32, best grid
+    index (N,) int32, every score (N, k) f32 or None)."""
+    if queries.dtype != DTYPE_F32:
+        raise ValueError("local rescoring scores FP32 queries")
+    require_device(quads, "quads")
+    qdt = {torch.float32: DTYPE_F32, torch.float16: DTYPE_F16}.get(quads.dtype)
+    if qdt is None or quads.dim() != 3 or quads.shape[2] != 4 or not quads.is_contiguous():
+        raise ValueError("quads must be the (S, C, 4) table of mp_quad_table")
+    rays = _f32_on_device(rays, "rays")
+    centers = _f32_on_device(centers, "centers")
+    grid = _f32_on_device(grid, "grid")
+    n = queries.rows
+    if rays.shape != (queries.n_pix, 3) or centers.shape != (n, 4) or grid.dim() != 2 or grid.shape[1] != 4:
+        raise ValueError("rays (D, 3), centers (N, 4) and grid (G, 4) must match the queries")
+    if cand.dim() != 2 or cand.shape[0] != n or not 1 <= cand.shape[1] <= 64:
+        raise ValueError("cand must be (N, k) with 1 <= k <= 64")
+    require_device(cand, "cand")
+    cand = cand.to(torch.int32).contiguous()
+    dev = centers.device
+    k = cand.shape[1]
+    best_val = torch.empty(n, dtype=torch.float32, device=dev)
+    best_idx = torch.empty(n, dtype=torch.int32, device=dev)
+    scores = torch.empty((n, k), dtype=torch.float32, device=dev) if all_scores else None
+    st = _lib.load().ebsd_local_rescore(quads.data_ptr(), qdt, quads.shape[0], quads.shape[1], rays.data_ptr(),
+                                        queries.n_pix, queries.data.data_ptr(), n, queries.ld, centers.data_ptr(),
+                                        grid.data_ptr(), grid.shape[0], cand.data_ptr(), k,
+                                        scores.data_ptr() if scores is not None else None, best_val.data_ptr(),
+                                        best_idx.data_ptr(), _lib.stream_ptr(dev))
+    check(st, "ebsd_local_rescore")
+    return best_val, best_idx, scores
+
+
 def rot6d_to_quat(z6: torch.Tensor) -> torch.Tensor:
     """(B, >= 6) network outputs -> (B, 4) unit quaternions, w >= 0, of the
     Gram-Schmidt rotation the geodesic loss scores."""

@@ -418,13 +418,19 @@ class LearnedIndexer:
                  fallback=None, threshold: float = 0.0, grid: Optional[torch.Tensor] = None,
                  radius_deg: float = 3.0, radius_steps: int = 6, shell_fallback: bool = True,
                  binning: int = 1, local_rays: Optional[torch.Tensor] = None,
-                 quad_dtype: torch.dtype = torch.float32):
+                 quad_dtype: torch.dtype = torch.float32, rescore_k: int = 8):
         self.net = net
         # The local refinement may score virtually binned patterns
         # (average-pooled binning x binning, as ebsd/refining.py's
         # virtual_binning with get_patterns(binning=...)) against the binned
         # detector's rays; the network and the fallback see full patterns.
+        # With binning, the best rescore_k binned candidates are rescored on
+        # the full patterns and the best full-resolution cosine is kept (0:
+        # keep the best binned candidate and its binned cosine).
         self.binning = int(binning)
+        self.rescore_k = int(rescore_k)
+        if not 0 <= self.rescore_k <= 16:
+            raise ValueError("rescore_k must be in [0, 16] (the local search keeps at most 16 candidates)")
         if self.binning > 1:
             if local_rays is None:
                 raise ValueError("binning > 1 needs the binned detector's rays (geometry.get_rays((b, b)))")
@@ -474,11 +480,19 @@ class LearnedIndexer:
         (orientation (B, 4), cosine (B,), grid index (B,), proposal (B, 4))."""
         q0 = self.net.propose(x)
         xm = self._local_queries(x)
-        val, idx = kernels.local_refine(self.quads, self.local_rays, xm, q0, self.grid, topk=1)
-        j = idx[:, 0].long()
+        if self.binning > 1 and self.rescore_k > 0:
+            k = min(self.rescore_k, self.grid.shape[0])
+            _, cand = kernels.local_refine(self.quads, self.local_rays, xm, q0, self.grid, topk=k)
+            full = kernels.PatternMatrix(x.contiguous(), None, DTYPE_F32, x.shape[1])
+            v, j, _ = kernels.local_rescore(self.quads, self.rays, full, q0, self.grid, cand)
+            # -1 only when no candidate was found (score -inf: falls back)
+            j = j.long().clamp_min(0)
+        else:
+            val, idx = kernels.local_refine(self.quads, self.local_rays, xm, q0, self.grid, topk=1)
+            v, j = val[:, 0], idx[:, 0].long()
         q = qu_prod(self.grid[j], q0)
         q = q / q.norm(dim=1, keepdim=True)
-        return q, val[:, 0], j, q0
+        return q, v, j, q0
 
     def index(self, patterns: torch.Tensor, chunk: int = 8192):
         """patterns (N, D) float32 on the GPU -> dict with orientations (N, 4)

@@ -310,6 +310,28 @@ int ebsd_local_refine(const void* quads, int quad_dtype, int mp_rows, int mp_col
                       const float* centers, const float* grid, int n_grid,
                       int topk, float* out_val, int* out_idx, void* stream);
 
+/*
+ * Rescoring of listed local candidates: for each query row i and each of its
+ * k candidates cand[i][c] (grid indices, -1 = none), the cosine of
+ * qu_prod(grid[cand[i][c]], centers[i]) projected as in ebsd_local_refine,
+ * against the query at this call's resolution (rays, n_pix). Typically the
+ * top-k of ebsd_local_refine on binned patterns, rescored on the full
+ * patterns. Writes best_val[i] / best_idx[i] (the best candidate, higher
+ * score then lower list position; -inf / -1 when no candidate is valid) and,
+ * if out_val is not NULL, every score out_val (m, k) (-inf for invalid
+ * entries).
+ *   cand (m, k) int32, 1 <= k <= 64; other operands as in ebsd_local_refine
+ *   (no limit on n_grid here)
+ * No counterpart in the reference: its refinement (ebsd/refining.py:104-135)
+ * scores candidates at the virtually binned resolution only; this is the
+ * same candidate projection and cosine, at the detector's own size.
+ */
+int ebsd_local_rescore(const void* quads, int quad_dtype, int mp_rows, int mp_cols,
+                       const float* rays, int n_pix, const float* q, int m, int ld,
+                       const float* centers, const float* grid, int n_grid,
+                       const int* cand, int k, float* out_val, float* best_val,
+                       int* best_idx, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -306,6 +306,87 @@ def test_local_refine_matches_oracle(small_mp, det, qdt):
         assert bool((val[i, 1:] <= val[i, :-1]).all())
 
 
+@pytest.mark.parametrize("det,k,qdt", [(20, 1, torch.float32), (20, 5, torch.float32), (35, 13, torch.float32),
+                                       (20, 64, torch.float32), (20, 8, torch.float16)])
+def test_local_rescore_matches_oracle(small_mp, det, k, qdt):
+    K = _kernels()
+    from ebsdtorch_amd.orientations import local_ball_grid_quats, qu_prod
+
+    n = 6
+    rays = O.normalized_rays((det, det), (0.5, 0.5, 0.6))
+    truth = _rand_quats(n, 70 + det + k)
+    pats = O.project_dictionary(small_mp, rays, truth)
+    g = torch.Generator().manual_seed(det + k)
+    pats = pats + 0.02 * torch.randn(pats.shape, generator=g)
+    x = pats - pats.mean(1, keepdim=True)
+    x = (x / x.norm(dim=1, keepdim=True)).float()
+    pert = torch.nn.functional.normalize(torch.tensor([[1.0, 0.006, -0.004, 0.005]]).expand(n, 4), dim=1)
+    centers = qu_prod(pert, truth)
+    centers = (centers / centers.norm(dim=1, keepdim=True)).float()
+    grid = local_ball_grid_quats(3.0, 3)
+    ref = L.local_scores(small_mp, rays, x.double(), centers, grid)
+    # random candidate lists with repeats; row 1 holds -1 entries, row 2 only -1
+    cand = torch.randint(0, grid.shape[0], (n, k), generator=g)
+    cand[1, ::2] = -1
+    cand[2] = -1
+    if k == 1:
+        cand[1] = -1
+    # query rows padded to a longer row stride
+    ld = det * det + 7
+    xp = torch.zeros(n, ld)
+    xp[:, :det * det] = x
+    xm = K.PatternMatrix(xp.to(DEV), None, 0, det * det)
+    print("local_rescore", det, k, qdt, flush=True)
+    quads = K.mp_quad_table(small_mp.to(DEV), qdt)
+    bv, bi, sc = K.local_rescore(quads, rays.to(DEV), xm, centers.to(DEV), grid.to(DEV), cand.to(DEV),
+                                 all_scores=True)
+    bv, bi, sc = bv.cpu().double(), bi.cpu().long(), sc.cpu().double()
+    tol = 2e-5 if qdt == torch.float32 else 2e-3
+    for i in range(n):
+        valid = cand[i] >= 0
+        assert bool(torch.isinf(sc[i][~valid]).all()) and bool((sc[i][~valid] < 0).all())
+        if valid.any():
+            torch.testing.assert_close(sc[i][valid], ref[i, cand[i][valid]], atol=tol, rtol=0)
+            # the best is the first listed candidate with the maximal score
+            pos = int(torch.nonzero(sc[i] == sc[i].max())[0])
+            assert int(bi[i]) == int(cand[i, pos]) and float(bv[i]) == float(sc[i, pos])
+            assert float(ref[i, bi[i]]) >= float(ref[i, cand[i][valid]].max()) - tol
+        else:
+            assert int(bi[i]) == -1 and float(bv[i]) == float("-inf")
+
+
+def test_binned_local_search_rescored_at_full_resolution():
+    """With binning, LearnedIndexer.local returns the best full-resolution
+    cosine among the binned search's top rescore_k candidates."""
+    from ebsdtorch_amd.learned import LearnedIndexer, ProposalNet
+    from ebsdtorch_amd.geometry import EBSDGeometry
+    from ebsdtorch_amd.orientations import local_ball_grid_quats
+    from ebsdtorch_amd.synthetic import band_master_pattern
+
+    K = _kernels()
+    geo = EBSDGeometry((32, 32), proj_center=(0.5, 0.5, 0.6))
+    mp = band_master_pattern(41, laue_id=11, n_families=4, seed=1)
+    net = ProposalNet((32, 32), DEV, laue_id=11, compute="fp32", channels=(8, 16, 32, 32), hidden=64)
+    grid = local_ball_grid_quats(3.0, 2)
+    rays = geo.get_rays()
+    li = LearnedIndexer(net, mp.to(DEV), rays.to(DEV), 11, grid=grid, binning=2,
+                        local_rays=geo.get_rays((2, 2)).to(DEV), rescore_k=6)
+    truth = _rand_quats(5, 91)
+    pats = O.project_dictionary(mp, rays, truth)
+    x = pats - pats.mean(1, keepdim=True)
+    x = (x / x.norm(dim=1, keepdim=True)).float().to(DEV)
+    print("binned rescore", flush=True)
+    q, v, j, q0 = li.local(x)
+    xm = li._local_queries(x)
+    _, cand = K.local_refine(li.quads, li.local_rays, xm, q0, li.grid, topk=6)
+    ref = L.local_scores(mp, rays, x.cpu().double(), q0.cpu(), grid)
+    cand, j, v = cand.cpu().long(), j.cpu().long(), v.cpu().double()
+    for i in range(5):
+        assert int(j[i]) in cand[i].tolist()
+        torch.testing.assert_close(v[i], ref[i, j[i]], atol=2e-5, rtol=0)
+        assert float(ref[i, j[i]]) >= float(ref[i, cand[i]].max()) - 2e-5
+
+
 def test_binned_local_queries_match_avg_pool():
     from ebsdtorch_amd.learned import LearnedIndexer, ProposalNet
     from ebsdtorch_amd.geometry import EBSDGeometry
