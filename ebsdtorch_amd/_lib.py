@@ -99,6 +99,15 @@ _SIGNATURES = {
     "ebsd_local_rescore": (
         _c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_void_p,
                  _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "ebsd_nn_conv_c1": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_int, ctypes.c_float, _c_void_p, _c_int, _c_int, _c_void_p, _c_int,
+                 _c_void_p, _c_void_p]),
+    "ebsd_nn_conv_gemm": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_void_p,
+                 _c_void_p, _c_void_p]),
+    "ebsd_gemm_nt_bias_act": (
+        _c_int, [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int,
+                 _c_void_p]),
     "ebsd_nn_col2im": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p]),
     "ebsd_nn_bias_act": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p]),
     "ebsd_nn_relu_bwd": (_c_int, [_c_void_p, _c_void_p, ctypes.c_longlong, _c_void_p]),

@@ -2,13 +2,18 @@
 //
 // The network is 4 convolutions (3x3, stride 2, padding 1, ReLU) and a
 // 2-layer MLP producing a 6D rotation. Activations are NHWC float32;
-// convolutions run as im2col + MFMA GEMM (ebsd_gemm_nt), so this file holds
-// the data-movement and element-wise pieces around the GEMMs: im2col /
-// col2im, bias + ReLU forward and backward, column sums (bias gradients),
+// convolutions run as im2col + MFMA GEMM with a bias + ReLU epilogue
+// (ebsd_gemm_nt_bias_act), except the first (one input channel, 9 taps per
+// output), which is computed directly. This file holds the data-movement and
+// element-wise pieces around the GEMMs: the direct first convolution, im2col
+// / col2im, bias + ReLU forward and backward, column sums (bias gradients),
 // transpose-and-cast (GEMM operand layouts), the symmetric geodesic loss and
 // the Adam update.
 
+#include <algorithm>
+
 #include "common.h"
+#include "mfma_tile.h"
 
 #include <hip/hip_bf16.h>
 
@@ -46,6 +51,271 @@ __global__ __launch_bounds__(256) void im2col_kernel(const float* __restrict__ X
   }
   col[idx] = cast_out<T>(v);
   }
+}
+
+// Row-major im2col (the forward GEMM's A operand) with one thread per 8
+// consecutive columns of one row and 32-bit index arithmetic (the entry
+// point checks rows * ld / 8 < 2^31): with C a multiple of 8 the 8 columns
+// are 8 consecutive channels of one tap, two 16-byte loads; otherwise (the
+// first convolution, C = 1) they are gathered one by one. Same values as
+// im2col_kernel<T, false>.
+template <typename T>
+__device__ __forceinline__ void store8(T* dst, const float (&v)[8]);
+template <>
+__device__ __forceinline__ void store8<float>(float* dst, const float (&v)[8]) {
+  reinterpret_cast<float4*>(dst)[0] = make_float4(v[0], v[1], v[2], v[3]);
+  reinterpret_cast<float4*>(dst)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+template <>
+__device__ __forceinline__ void store8<__hip_bfloat16>(__hip_bfloat16* dst, const float (&v)[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    w[e] = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * e])) |
+           ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * e + 1])) << 16);
+  *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void im2col8_kernel(const float* __restrict__ X, int H, int W, int C, int Ho,
+                                                      int Wo, float scale, T* __restrict__ col, int ld,
+                                                      int n_units) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n_units) return;
+  const int ld8 = ld >> 3;
+  const int row = t / ld8;
+  const int k0 = (t - row * ld8) << 3;
+  const int ox = row % Wo;
+  const int r2 = row / Wo;
+  const int oy = r2 % Ho;
+  const int b = r2 / Ho;
+  float v[8];
+  if ((C & 7) == 0) {
+    const int kk = k0 / C;
+    const int c = k0 - kk * C;
+    const int ky = kk / 3, kx = kk - 3 * (kk / 3);
+    const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+    if (kk < 9 && y >= 0 && y < H && x >= 0 && x < W) {
+      const float4* p = reinterpret_cast<const float4*>(X + (((int64_t)b * H + y) * W + x) * C + c);
+      const float4 a = p[0], d = p[1];
+      v[0] = scale * a.x; v[1] = scale * a.y; v[2] = scale * a.z; v[3] = scale * a.w;
+      v[4] = scale * d.x; v[5] = scale * d.y; v[6] = scale * d.z; v[7] = scale * d.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = 0.0f;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + e;
+      const int kk = k / C;
+      const int c = k - kk * C;
+      const int ky = kk / 3, kx = kk - 3 * (kk / 3);
+      const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+      v[e] = (kk < 9 && y >= 0 && y < H && x >= 0 && x < W)
+                 ? scale * X[(((int64_t)b * H + y) * W + x) * C + c]
+                 : 0.0f;
+    }
+  }
+  store8<T>(col + (int64_t)t * 8, v);
+}
+
+// Direct convolution of a one-channel image (the first layer) with fused
+// bias + ReLU: Y[row][co] = max(0, bias[co] + sum_t w[co][t] in_t) over the
+// 9 taps t = ky*3 + kx, in_t = round_T(scale * X[2 oy + ky - 1][2 ox + kx - 1])
+// (0 outside the image): the values of im2col<T> + the GEMM + bias_act, up to
+// the order of the fp32 sum. One thread per (row, 4 output channels), so a
+// wave stores consecutive 16-byte pieces of the NHWC rows; the taps are read
+// by the c_out / 4 threads of a row from cache. Weights and biases are staged
+// in LDS as 12 floats per channel (9 taps, bias, 2 pad), once per workgroup
+// of a grid-stride launch (8 workgroups per CU).
+template <typename T>
+__device__ __forceinline__ float round_to(float v);
+template <>
+__device__ __forceinline__ float round_to<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float round_to<__hip_bfloat16>(float v) { return __bfloat162float(__float2bfloat16(v)); }
+
+template <typename T>
+__device__ __forceinline__ float load_float(const T* p);
+template <>
+__device__ __forceinline__ float load_float<float>(const float* p) { return *p; }
+template <>
+__device__ __forceinline__ float load_float<__hip_bfloat16>(const __hip_bfloat16* p) { return __bfloat162float(*p); }
+
+constexpr int kConvC1MaxOut = 256;
+
+template <typename T>
+__global__ __launch_bounds__(256) void conv_c1_kernel(const float* __restrict__ X, int H, int W, int Ho, int Wo,
+                                                      float scale, const T* __restrict__ wgt, int ld_w,
+                                                      const float* __restrict__ bias, int c_out,
+                                                      float* __restrict__ Y, int n_units) {
+  __shared__ __attribute__((aligned(16))) float ws[kConvC1MaxOut * 12];
+  for (int e = threadIdx.x; e < c_out * 12; e += 256) {
+    const int co = e / 12, t = e - 12 * (e / 12);
+    ws[e] = t < 9 ? load_float<T>(wgt + (size_t)co * ld_w + t) : (t == 9 ? bias[co] : 0.0f);
+  }
+  __syncthreads();
+  const int q4 = c_out >> 2;
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < n_units; t += gridDim.x * 256) {
+  const int row = t / q4;
+  const int co = (t - row * q4) << 2;
+  const int ox = row % Wo;
+  const int r2 = row / Wo;
+  const int oy = r2 % Ho;
+  const int b = r2 / Ho;
+  const float* __restrict__ img = X + (int64_t)b * H * W;
+  float in[9];
+#pragma unroll
+  for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx) {
+      const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+      in[ky * 3 + kx] = (y >= 0 && y < H && x >= 0 && x < W) ? round_to<T>(scale * img[y * W + x]) : 0.0f;
+    }
+  float r[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4* wp = reinterpret_cast<const float4*>(ws + (co + q) * 12);
+    const float4 a = wp[0], c = wp[1], d = wp[2];
+    float s = a.x * in[0];
+    s = fmaf(a.y, in[1], s);
+    s = fmaf(a.z, in[2], s);
+    s = fmaf(a.w, in[3], s);
+    s = fmaf(c.x, in[4], s);
+    s = fmaf(c.y, in[5], s);
+    s = fmaf(c.z, in[6], s);
+    s = fmaf(c.w, in[7], s);
+    s = fmaf(d.x, in[8], s);
+    r[q] = fmaxf(s + d.y, 0.0f);
+  }
+  *reinterpret_cast<float4*>(Y + (int64_t)row * c_out + co) = make_float4(r[0], r[1], r[2], r[3]);
+  }
+}
+
+// Implicit-GEMM convolution (3x3, stride 2, padding 1) with bias + ReLU:
+// Y[row][co] = max(0, bias[co] + sum_k A[row][k] Wt[co][k]) where A is the
+// im2col matrix of the NHWC float32 activations X, which is never stored:
+// each K stage of the 128 x 128 MFMA tile (the GEMM's LDS layout and K order)
+// is filled straight from X, converted to the operand type as im2col does,
+// so Y equals im2col + the GEMM + bias/ReLU bit for bit. C must be a
+// multiple of the 16-byte run of a stage (8 BF16 / 4 FP32 channels), so a
+// thread's run is consecutive channels of one tap; K columns past 9 C read
+// as zeros (the weights are zero there too).
+template <int DT>
+__device__ __forceinline__ ebsd_tile::floatx4 conv_a_run(const float* __restrict__ X, bool valid, int64_t off) {
+  using ebsd_tile::floatx4;
+  if constexpr (DT == EBSD_DTYPE_F32) {
+    if (!valid) return floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    return *reinterpret_cast<const floatx4*>(X + off);
+  } else {
+    float v[8];
+    if (valid) {
+      const float4 a = reinterpret_cast<const float4*>(X + off)[0];
+      const float4 b = reinterpret_cast<const float4*>(X + off)[1];
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = 0.0f;
+    }
+    uint32_t w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      w[e] = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * e])) |
+             ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * e + 1])) << 16);
+    return __builtin_bit_cast(floatx4, make_uint4(w[0], w[1], w[2], w[3]));
+  }
+}
+
+template <int DT>
+__global__ __launch_bounds__(256, 2) void conv_gemm_kernel(const float* __restrict__ X, int H, int W, int C, int Ho,
+                                                           int Wo, int M, const unsigned char* __restrict__ Wt,
+                                                           int N, int ld_bytes, const float* __restrict__ bias,
+                                                           float* __restrict__ Y) {
+  using namespace ebsd_tile;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[LDS_BYTES];
+  constexpr int EB = DT == EBSD_DTYPE_F32 ? 4 : 2;
+  constexpr int STAGE_ELEMS = STAGE_BYTES / EB;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
+  const int lr = tid >> 3;
+  const int lc = (tid & 7) * 16;
+  const int ke = lc / EB;  // first K element of this thread's run within a stage
+  // the thread's 4 A rows: image base row b*H, top-left input corner
+  int rbh[4], ry[4], rx[4];
+  bool rv[4];
+  const unsigned char* brow[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = i0 + lr + 32 * i;
+    rv[i] = row < M;
+    const int r = min(row, M - 1);
+    const int ox = r % Wo;
+    const int t = r / Wo;
+    const int oy = t % Ho;
+    rbh[i] = (t / Ho) * H;
+    ry[i] = 2 * oy - 1;
+    rx[i] = 2 * ox - 1;
+    brow[i] = Wt + (size_t)min(j0 + lr + 32 * i, N - 1) * ld_bytes + lc;
+  }
+  const int nk = ld_bytes / STAGE_BYTES;
+  floatx4 ra[4], rb[4];
+  auto load = [&](int kt) {
+    const int k = kt * STAGE_ELEMS + ke;
+    const int tap = k / C;
+    const int c = k - tap * C;
+    const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int y = ry[i] + ky, x = rx[i] + kx;
+      const bool ok = rv[i] && tap < 9 && y >= 0 && y < H && x >= 0 && x < W;
+      ra[i] = conv_a_run<DT>(X, ok, ((int64_t)(rbh[i] + y) * W + x) * C + c);
+      rb[i] = *reinterpret_cast<const floatx4*>(brow[i] + (size_t)kt * STAGE_BYTES);
+    }
+  };
+  const int a_off = (lane & 31) * LDS_ROW + lane_k_off<DT>(lane);
+  const int b_off = (wave * 32 + (lane & 31)) * LDS_ROW + lane_k_off<DT>(lane);
+  const int st_off = lr * LDS_ROW + lc;
+  floatx16 acc[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[a][r] = 0.0f;
+  load(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    *reinterpret_cast<floatx4*>(smem + st_off + 32 * i * LDS_ROW) = ra[i];
+    *reinterpret_cast<floatx4*>(smem + LDS_TILE + st_off + 32 * i * LDS_ROW) = rb[i];
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) load(kt + 1);
+    const unsigned char* As = smem + cur * 2 * LDS_TILE;
+    stage_mfma<DT>(As, As + LDS_TILE, a_off, b_off, acc);
+    if (more) {
+      unsigned char* Ad = smem + (cur ^ 1) * 2 * LDS_TILE;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        *reinterpret_cast<floatx4*>(Ad + st_off + 32 * i * LDS_ROW) = ra[i];
+        *reinterpret_cast<floatx4*>(Ad + LDS_TILE + st_off + 32 * i * LDS_ROW) = rb[i];
+      }
+    }
+    __syncthreads();
+  }
+  const int col = j0 + wave * 32 + (lane & 31);
+  if (col >= N) return;
+  const float bj = bias[col];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = i0 + bi * 32 + mfma32_row(r, lane);
+      if (row < M) Y[(size_t)row * N + col] = fmaxf(acc[bi][r] + bj, 0.0f);
+    }
 }
 
 // dst[r][c] = src[r][c] for c < cols, 0 for cols <= c < ld_dst.
@@ -338,7 +608,21 @@ extern "C" int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, float 
   if (transposed ? (ld < n_rows) : (ld < 9 * c)) return EBSD_ERR_SHAPE;
   const int64_t total = transposed ? (int64_t)9 * c * ld : n_rows * ld;
   hipStream_t st = static_cast<hipStream_t>(stream);
-#define EBSD_IM2COL(T, TR)                                                                                   \
+  const bool vec = !transposed && (ld & 7) == 0 && total / 8 < INT_MAX &&
+                   (reinterpret_cast<uintptr_t>(col) & 15) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+  if (vec && (dtype == EBSD_DTYPE_F32 || dtype == EBSD_DTYPE_BF16)) {
+    const int units = (int)(total / 8);
+    const dim3 g((unsigned)((units + 255) / 256));
+    if (dtype == EBSD_DTYPE_F32)
+      hipLaunchKernelGGL(im2col8_kernel<float>, g, dim3(256), 0, st, x, h, w, c, ho, wo, scale,
+                         static_cast<float*>(col), ld, units);
+    else
+      hipLaunchKernelGGL(im2col8_kernel<__hip_bfloat16>, g, dim3(256), 0, st, x, h, w, c, ho, wo, scale,
+                         static_cast<__hip_bfloat16*>(col), ld, units);
+    EBSD_LAUNCH_CHECK();
+    return EBSD_OK;
+  }
+#define EBSD_IM2COL(T, TR)                                                                                 \
   hipLaunchKernelGGL((im2col_kernel<T, TR>), grid1d(total), dim3(256), 0, st, x, b, h, w, c, ho, wo, scale,  \
                      static_cast<T*>(col), ld)
   if (dtype == EBSD_DTYPE_F32) {
@@ -349,6 +633,55 @@ extern "C" int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, float 
     return EBSD_ERR_DTYPE;
   }
 #undef EBSD_IM2COL
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_nn_conv_c1(const float* x, int b, int h, int w, float scale, const void* weight, int ld_w,
+                               int dtype, const float* bias, int c_out, float* y, void* stream) {
+  if (!x || !weight || !bias || !y) return EBSD_ERR_NULL;
+  if (b < 1 || h < 1 || w < 1 || ld_w < 9 || c_out < 4 || c_out > kConvC1MaxOut || (c_out & 3) != 0)
+    return EBSD_ERR_SHAPE;
+  if ((reinterpret_cast<uintptr_t>(y) & 15) != 0) return EBSD_ERR_ALIGN;
+  const int ho = (h + 1) / 2, wo = (w + 1) / 2;
+  const int64_t units = (int64_t)b * ho * wo * (c_out / 4);
+  if (units >= INT_MAX - (1 << 20)) return EBSD_ERR_SHAPE;
+  const dim3 g((unsigned)std::min<int64_t>((units + 255) / 256, 2048));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == EBSD_DTYPE_F32)
+    hipLaunchKernelGGL(conv_c1_kernel<float>, g, dim3(256), 0, st, x, h, w, ho, wo, scale,
+                       static_cast<const float*>(weight), ld_w, bias, c_out, y, (int)units);
+  else if (dtype == EBSD_DTYPE_BF16)
+    hipLaunchKernelGGL(conv_c1_kernel<__hip_bfloat16>, g, dim3(256), 0, st, x, h, w, ho, wo, scale,
+                       static_cast<const __hip_bfloat16*>(weight), ld_w, bias, c_out, y, (int)units);
+  else
+    return EBSD_ERR_DTYPE;
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_nn_conv_gemm(const float* x, int b, int h, int w, int c, const void* weight, int ld_w, int c_out,
+                                 int dtype, const float* bias, float* y, void* stream) {
+  if (!x || !weight || !bias || !y) return EBSD_ERR_NULL;
+  if (dtype != EBSD_DTYPE_F32 && dtype != EBSD_DTYPE_BF16) return EBSD_ERR_DTYPE;
+  const int run = dtype == EBSD_DTYPE_F32 ? 4 : 8;
+  if (b < 1 || h < 1 || w < 1 || c < 1 || c % run != 0 || c_out < 1) return EBSD_ERR_SHAPE;
+  if (ld_w < 9 * c || ld_w % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
+  if (((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(weight)) & 15) != 0) return EBSD_ERR_ALIGN;
+  const int ho = (h + 1) / 2, wo = (w + 1) / 2;
+  const int64_t m = (int64_t)b * ho * wo;
+  if (m >= INT_MAX || (int64_t)b * h >= INT_MAX) return EBSD_ERR_SHAPE;
+  const dim3 grid((unsigned)((m + ebsd_tile::TILE - 1) / ebsd_tile::TILE),
+                  (unsigned)((c_out + ebsd_tile::TILE - 1) / ebsd_tile::TILE));
+  const int eb = dtype == EBSD_DTYPE_F32 ? 4 : 2;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const unsigned char* wt = static_cast<const unsigned char*>(weight);
+  if (dtype == EBSD_DTYPE_F32)
+    hipLaunchKernelGGL(conv_gemm_kernel<EBSD_DTYPE_F32>, grid, dim3(256), 0, st, x, h, w, c, ho, wo, (int)m, wt,
+                       c_out, ld_w * eb, bias, y);
+  else
+    hipLaunchKernelGGL(conv_gemm_kernel<EBSD_DTYPE_BF16>, grid, dim3(256), 0, st, x, h, w, c, ho, wo, (int)m, wt,
+                       c_out, ld_w * eb, bias, y);
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;
 }

@@ -78,9 +78,10 @@ __global__ __launch_bounds__(256) void center_transpose_kernel(const float* __re
   }
 }
 
-// C[i][j] = alpha * (A B^T)[i][j] + beta * C[i][j] + w * u[i] * v[j]
-// for the 128 x 128 block (blockIdx.x, blockIdx.y). upper_only skips blocks
-// strictly below the block diagonal.
+// C[i][j] = act(alpha * (A B^T)[i][j] + beta * C[i][j] + w * u[i] * v[j] + bias[j])
+// for the 128 x 128 block (blockIdx.x, blockIdx.y), act = ReLU if relu (bias
+// NULL: no bias term). upper_only skips blocks strictly below the block
+// diagonal.
 // Split K: blockIdx.z = s covers stages [s * split_stages, (s + 1) *
 // split_stages) of the K loop and writes its own output slice
 // C + s * split_stride (alpha, beta, u, v apply per slice; the split-K path
@@ -91,7 +92,8 @@ __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(const unsigned char* __
                                                          float* __restrict__ C, int ldc, float alpha, float beta,
                                                          const float* __restrict__ u, const float* __restrict__ v,
                                                          float w, int upper_only, int split_stages,
-                                                         int64_t split_stride) {
+                                                         int64_t split_stride, const float* __restrict__ bias,
+                                                         int relu) {
   __shared__ __attribute__((aligned(16))) unsigned char smem[LDS_BYTES];
   const int bi_tile = blockIdx.x, bj_tile = blockIdx.y;
   if (upper_only && bi_tile > bj_tile) return;  // uniform over the workgroup
@@ -121,6 +123,7 @@ __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(const unsigned char* __
   const int col = j0 + wave * 32 + (lane & 31);
   if (col >= N) return;
   const float vj = (v && w != 0.0f) ? w * v[col] : 0.0f;
+  const float bj = bias ? bias[col] : 0.0f;
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi) {
 #pragma unroll
@@ -131,6 +134,8 @@ __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(const unsigned char* __
         float out = alpha * acc[bi][r];
         if (beta != 0.0f) out += beta * *cp;
         if (vj != 0.0f) out += u[row] * vj;
+        if (bias) out += bj;
+        if (relu) out = fmaxf(out, 0.0f);
         *cp = out;
       }
     }
@@ -161,7 +166,8 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restr
 
 int launch_gemm(const void* a, int m, const void* b, int n, int ld, int dtype, float* c, int ldc, float alpha,
                 float beta, const float* u, const float* v, float w, int upper_only, hipStream_t st,
-                int splits = 1, int split_stages = 0, int64_t split_stride = 0) {
+                int splits = 1, int split_stages = 0, int64_t split_stride = 0, const float* bias = nullptr,
+                int relu = 0) {
   const int eb = elem_bytes(dtype);
   if (splits <= 1) split_stages = ld * eb / STAGE_BYTES;
   const dim3 grid((m + TILE - 1) / TILE, (n + TILE - 1) / TILE, splits > 1 ? splits : 1), block(256);
@@ -170,15 +176,15 @@ int launch_gemm(const void* a, int m, const void* b, int n, int ld, int dtype, f
   switch (dtype) {
     case EBSD_DTYPE_F32:
       hipLaunchKernelGGL(gemm_nt_kernel<EBSD_DTYPE_F32>, grid, block, 0, st, A, m, B, n, ld * eb, c, ldc, alpha,
-                         beta, u, v, w, upper_only, split_stages, split_stride);
+                         beta, u, v, w, upper_only, split_stages, split_stride, bias, relu);
       break;
     case EBSD_DTYPE_BF16:
       hipLaunchKernelGGL(gemm_nt_kernel<EBSD_DTYPE_BF16>, grid, block, 0, st, A, m, B, n, ld * eb, c, ldc, alpha,
-                         beta, u, v, w, upper_only, split_stages, split_stride);
+                         beta, u, v, w, upper_only, split_stages, split_stride, bias, relu);
       break;
     case EBSD_DTYPE_F16:
       hipLaunchKernelGGL(gemm_nt_kernel<EBSD_DTYPE_F16>, grid, block, 0, st, A, m, B, n, ld * eb, c, ldc, alpha,
-                         beta, u, v, w, upper_only, split_stages, split_stride);
+                         beta, u, v, w, upper_only, split_stages, split_stride, bias, relu);
       break;
     default:
       return EBSD_ERR_DTYPE;
@@ -218,6 +224,19 @@ extern "C" int ebsd_gemm_nt(const void* a, int m, const void* b, int n, int ld, 
   if (m == 0 || n == 0) return EBSD_OK;
   return launch_gemm(a, m, b, n, ld, dtype, c, ldc, alpha, beta, nullptr, nullptr, 0.0f, 0,
                      static_cast<hipStream_t>(stream));
+}
+
+extern "C" int ebsd_gemm_nt_bias_act(const void* a, int m, const void* b, int n, int ld, int dtype, float* c,
+                                     int ldc, const float* bias, int relu, void* stream) {
+  if (!a || !b || !c || !bias) return EBSD_ERR_NULL;
+  if (m < 0 || n < 0 || ld < 1 || ldc < n) return EBSD_ERR_SHAPE;
+  const int eb = elem_bytes(dtype);
+  if (eb == 0) return EBSD_ERR_DTYPE;
+  if (ld % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
+  if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (m == 0 || n == 0) return EBSD_OK;
+  return launch_gemm(a, m, b, n, ld, dtype, c, ldc, 1.0f, 0.0f, nullptr, nullptr, 0.0f, 0,
+                     static_cast<hipStream_t>(stream), 1, 0, 0, bias, relu ? 1 : 0);
 }
 
 extern "C" int ebsd_gemm_nt_splitk_workspace(int m, int n, int splits, size_t* bytes) {

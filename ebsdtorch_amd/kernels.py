@@ -414,6 +414,86 @@ def transpose_pad(src: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def matmul_nt_bias_act(a: torch.Tensor, b: torch.Tensor, out: torch.Tensor, bias: torch.Tensor,
+                       relu: bool) -> torch.Tensor:
+    """out (m, n) = act(a b^T + bias) in one pass (ebsd_gemm_nt_bias_act);
+    operands as matmul_nt, act = ReLU if relu."""
+    if a.dtype != b.dtype or a.dim() != 2 or b.dim() != 2 or a.shape[1] != b.shape[1]:
+        raise ValueError("operands must be 2D with one dtype and one K")
+    if not (a.is_contiguous() and b.is_contiguous()):
+        raise ValueError("operands must be contiguous")
+    dt = _DT_OF.get(a.dtype)
+    if dt is None:
+        raise ValueError(f"unsupported operand dtype {a.dtype}")
+    m, n = a.shape[0], b.shape[0]
+    if out.dtype != torch.float32 or out.shape[0] != m or out.shape[1] < n or out.stride(1) != 1:
+        raise ValueError("output must be float32 (m, >= n) with unit column stride")
+    bias = _f32_on_device(bias, "bias")
+    if bias.numel() < n:
+        raise ValueError("bias needs one value per output column")
+    st = _lib.load().ebsd_gemm_nt_bias_act(a.data_ptr(), m, b.data_ptr(), n, a.shape[1], dt, out.data_ptr(),
+                                           out.stride(0), bias.data_ptr(), 1 if relu else 0,
+                                           _lib.stream_ptr(a.device))
+    check(st, "ebsd_gemm_nt_bias_act")
+    return out
+
+
+def conv_c1(x: torch.Tensor, weight: torch.Tensor, bias: torch.Tensor, scale: float = 1.0,
+            out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """First convolution (one input channel, 3x3 / stride 2 / padding 1) with
+    bias + ReLU, done directly (ebsd_nn_conv_c1): x (B, H, W, 1) or (B, H, W)
+    f32, weight (C_out, >= 9) f32/bf16 with columns ky*3 + kx -> (B*Ho*Wo,
+    C_out) f32."""
+    x = _f32_on_device(x, "x")
+    if x.dim() not in (3, 4) or (x.dim() == 4 and x.shape[3] != 1):
+        raise ValueError("conv_c1 takes one-channel images (B, H, W[, 1])")
+    b, h, w = x.shape[0], x.shape[1], x.shape[2]
+    dt = _DT_OF.get(weight.dtype)
+    if dt not in (DTYPE_F32, DTYPE_BF16) or weight.dim() != 2 or weight.stride(1) != 1 or weight.shape[1] < 9:
+        raise ValueError("weight must be (C_out, >= 9) float32/bfloat16 with unit column stride")
+    require_device(weight, "weight")
+    c_out = weight.shape[0]
+    bias = _f32_on_device(bias, "bias")
+    rows = b * ((h + 1) // 2) * ((w + 1) // 2)
+    if out is None:
+        out = torch.empty((rows, c_out), dtype=torch.float32, device=x.device)
+    if out.shape != (rows, c_out) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be contiguous float32 (B*Ho*Wo, C_out)")
+    st = _lib.load().ebsd_nn_conv_c1(x.data_ptr(), b, h, w, float(scale), weight.data_ptr(), weight.stride(0), dt,
+                                     bias.data_ptr(), c_out, out.data_ptr(), _lib.stream_ptr(x.device))
+    check(st, "ebsd_nn_conv_c1")
+    return out
+
+
+def conv_gemm(x: torch.Tensor, weight: torch.Tensor, bias: torch.Tensor,
+              out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """3x3 / stride 2 / padding 1 convolution + bias + ReLU as an implicit GEMM
+    (ebsd_nn_conv_gemm): x (B, H, W, C) f32 NHWC, weight (C_out, K_pad)
+    f32/bf16 with column (ky*3 + kx)*C + c, zero beyond 9C -> (B*Ho*Wo, C_out)
+    f32, equal to im2col + matmul_nt_bias_act."""
+    x = _f32_on_device(x, "x")
+    if x.dim() != 4:
+        raise ValueError("x must be NHWC (B, H, W, C)")
+    b, h, w, c = x.shape
+    dt = _DT_OF.get(weight.dtype)
+    if dt not in (DTYPE_F32, DTYPE_BF16) or weight.dim() != 2 or not weight.is_contiguous():
+        raise ValueError("weight must be a contiguous float32/bfloat16 (C_out, K_pad) matrix")
+    require_device(weight, "weight")
+    c_out = weight.shape[0]
+    bias = _f32_on_device(bias, "bias")
+    if bias.numel() < c_out:
+        raise ValueError("bias needs one value per output channel")
+    rows = b * ((h + 1) // 2) * ((w + 1) // 2)
+    if out is None:
+        out = torch.empty((rows, c_out), dtype=torch.float32, device=x.device)
+    if out.shape != (rows, c_out) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be contiguous float32 (B*Ho*Wo, C_out)")
+    st = _lib.load().ebsd_nn_conv_gemm(x.data_ptr(), b, h, w, c, weight.data_ptr(), weight.shape[1], c_out, dt,
+                                       bias.data_ptr(), out.data_ptr(), _lib.stream_ptr(x.device))
+    check(st, "ebsd_nn_conv_gemm")
+    return out
+
+
 def bias_act(y: torch.Tensor, bias: torch.Tensor, cols: int, relu: bool) -> torch.Tensor:
     """y[:, :cols] = act(y[:, :cols] + bias) in place."""
     require_device(y, "y")

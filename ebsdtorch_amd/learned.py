@@ -194,21 +194,28 @@ class ProposalNet:
         B = x.shape[0]
         acts = [x.view(B, self.H, self.W, 1)]
         for l, (h, w, ci, ho, wo, co) in enumerate(self.conv_shapes):
-            kp = self.params[f"conv{l}.w"].shape[1]
-            col = torch.empty((B * ho * wo, kp), dtype=self.cdtype, device=self.device)
-            kernels.im2col(acts[-1], col, scale=math.sqrt(self.n_pix) if l == 0 else 1.0)
-            y = torch.empty((B * ho * wo, co), dtype=torch.float32, device=self.device)
-            kernels.matmul_nt(col, self.weight_lp(f"conv{l}.w"), y)
-            del col
-            kernels.bias_act(y, self.view(f"conv{l}.b"), co, relu=True)
+            scale = math.sqrt(self.n_pix) if l == 0 else 1.0
+            if ci == 1 and co % 4 == 0 and co <= 256:
+                # one input channel: 9 taps per output, done directly
+                y = kernels.conv_c1(acts[-1], self.weight_lp(f"conv{l}.w"), self.view(f"conv{l}.b"), scale)
+            elif scale == 1.0 and ci % (4 if self.cdtype == torch.float32 else 8) == 0:
+                # implicit GEMM: the im2col operand is built in LDS, never stored
+                y = kernels.conv_gemm(acts[-1], self.weight_lp(f"conv{l}.w"), self.view(f"conv{l}.b"))
+            else:
+                kp = self.params[f"conv{l}.w"].shape[1]
+                col = torch.empty((B * ho * wo, kp), dtype=self.cdtype, device=self.device)
+                kernels.im2col(acts[-1], col, scale=scale)
+                y = torch.empty((B * ho * wo, co), dtype=torch.float32, device=self.device)
+                kernels.matmul_nt_bias_act(col, self.weight_lp(f"conv{l}.w"), y, self.view(f"conv{l}.b"), True)
+                del col
             acts.append(y.view(B, ho, wo, co))
         flat = acts[-1].view(B, self.feat)
         h1 = torch.empty((B, self.hidden), dtype=torch.float32, device=self.device)
-        kernels.matmul_nt(self._operand(flat, self.feat), self.weight_lp("fc1.w"), h1)
-        kernels.bias_act(h1, self.view("fc1.b"), self.hidden, relu=True)
+        kernels.matmul_nt_bias_act(self._operand(flat, self.feat), self.weight_lp("fc1.w"), h1,
+                                   self.view("fc1.b"), True)
         z = torch.empty((B, 6), dtype=torch.float32, device=self.device)
-        kernels.matmul_nt(self._operand(h1, self.hidden), self.weight_lp("fc2.w"), z)
-        kernels.bias_act(z, self.view("fc2.b"), 6, relu=False)
+        kernels.matmul_nt_bias_act(self._operand(h1, self.hidden), self.weight_lp("fc2.w"), z,
+                                   self.view("fc2.b"), False)
         if keep:
             self._acts = acts
             self._h1 = h1

@@ -239,6 +239,22 @@ int ebsd_cov_update(const float* x, int n, int d, int ld_x, float* mean,
  *                      column (ky*3 + kx)*c + ch, values times scale, zero
  *                      padding to ld; transposed != 0 stores col^T
  *                      (9c, ld), ld >= b*ho*wo, zero columns beyond
+ *   ebsd_nn_conv_c1    first convolution (one input channel) done directly:
+ *                      y (b*ho*wo, c_out) f32 = ReLU(bias + w . taps), taps
+ *                      scale * x rounded to dtype (F32/BF16), weight
+ *                      (c_out, ld_w) of dtype, columns ky*3 + kx; the values
+ *                      of im2col + GEMM + bias_act up to the fp32 sum order;
+ *                      c_out a multiple of 4, <= 256, y 16-byte aligned
+ *   ebsd_nn_conv_gemm  convolution as an implicit GEMM with bias + ReLU: the
+ *                      im2col operand of x (b, h, w, c) f32 is built in LDS
+ *                      per K stage, never stored; y (b*ho*wo, c_out) f32
+ *                      equals ebsd_nn_im2col + ebsd_gemm_nt_bias_act bit for
+ *                      bit. weight (c_out, ld_w) of dtype F32/BF16, ld_w >= 9c
+ *                      a multiple of ebsd_pad_multiple(dtype), zero beyond 9c;
+ *                      c a multiple of 4 (F32) / 8 (BF16); x, weight 16-byte
+ *                      aligned
+ *   ebsd_gemm_nt_bias_act  c = act(a b^T + bias[col]) (ebsd_gemm_nt's operand
+ *                      rules; act = ReLU if relu): GEMM + bias_act in one pass
  *   ebsd_nn_cast       dst (rows, ld_dst) F32/BF16 = src (rows, cols) f32,
  *                      zero columns beyond cols
  *   ebsd_nn_col2im     dcol (b*ho*wo, ld) f32 -> dx (b, h, w, c) f32 (gather)
@@ -254,6 +270,15 @@ int ebsd_cov_update(const float* x, int n, int d, int ld_x, float* mean,
  *   ebsd_nn_rot6d_to_quat  6D -> the loss's rotation R -> unit quaternion
  *                      (w >= 0) with quat_matrix(q) = R, (b, 4) f32
  */
+int ebsd_nn_conv_c1(const float* x, int b, int h, int w, float scale,
+                    const void* weight, int ld_w, int dtype, const float* bias,
+                    int c_out, float* y, void* stream);
+int ebsd_nn_conv_gemm(const float* x, int b, int h, int w, int c,
+                      const void* weight, int ld_w, int c_out, int dtype,
+                      const float* bias, float* y, void* stream);
+int ebsd_gemm_nt_bias_act(const void* a, int m, const void* b, int n, int ld,
+                          int dtype, float* c, int ldc, const float* bias,
+                          int relu, void* stream);
 int ebsd_nn_im2col(const float* x, int b, int h, int w, int c, float scale,
                    void* col, int ld, int dtype, int transposed, void* stream);
 int ebsd_nn_cast(const float* src, int rows, int cols, int ld_src, void* dst,

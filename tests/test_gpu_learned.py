@@ -96,6 +96,90 @@ def test_im2col_matches_unfold_and_transposed():
     assert float(colt[:, ref.shape[0]:].float().abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("dtype,ld", [(torch.float32, 144), (torch.bfloat16, 192), (torch.float32, 150)])
+def test_im2col_channel_vectors_match_unfold(dtype, ld):
+    """C a multiple of 8 (whole 8-channel runs of one tap per thread), on
+    padded and unpadded rows; ld 150 (not a multiple of 8) takes the generic
+    kernel."""
+    K = _kernels()
+    g = torch.Generator().manual_seed(5)
+    x = torch.randn(3, 11, 10, 16, generator=g)
+    ref = torch.nn.functional.unfold(x.permute(0, 3, 1, 2), 3, padding=1, stride=2)
+    ref = ref.view(3, 16, 9, -1).permute(0, 3, 2, 1).reshape(-1, 144)
+    col = torch.full((ref.shape[0], ld), 7.0, device=DEV, dtype=dtype)
+    print("im2col C=16", dtype, ld, flush=True)
+    K.im2col(x.to(DEV), col, scale=0.5)
+    torch.testing.assert_close(col[:, :144].float().cpu(), (0.5 * ref).to(dtype).float(), atol=0, rtol=0)
+    if ld > 144:
+        assert float(col[:, 144:].float().abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_conv_c1_matches_im2col_gemm(dtype):
+    """The direct first convolution equals im2col + GEMM + bias/ReLU up to the
+    order of the 9-term fp32 sum (bf16 taps and weights: exact products), and
+    the fused GEMM epilogue equals GEMM then bias_act bit for bit."""
+    K = _kernels()
+    g = torch.Generator().manual_seed(6)
+    x = torch.randn(3, 13, 10, 1, generator=g)
+    co = 32
+    kp = 64 if dtype == torch.bfloat16 else 32
+    wt = torch.zeros(co, kp)
+    wt[:, :9] = torch.randn(co, 9, generator=g)
+    bias = 0.3 * torch.randn(co, generator=g)
+    wd = wt.to(dtype).to(DEV)
+    print("conv_c1", dtype, flush=True)
+    y = K.conv_c1(x.to(DEV), wd, bias.to(DEV), scale=1.7).cpu().double()
+    col = torch.empty((3 * 7 * 5, kp), dtype=dtype, device=DEV)
+    K.im2col(x.to(DEV), col, scale=1.7)
+    ref = torch.empty((col.shape[0], co), device=DEV)
+    K.matmul_nt(col, wd, ref)
+    K.bias_act(ref, bias.to(DEV), co, relu=True)
+    torch.testing.assert_close(y, ref.cpu().double(), atol=2e-6, rtol=1e-6)
+    # against float64 on the rounded operands
+    taps = torch.nn.functional.unfold((1.7 * x).to(dtype).double().permute(0, 3, 1, 2), 3, padding=1, stride=2)
+    taps = taps.permute(0, 2, 1).reshape(-1, 9)
+    ref64 = torch.relu(taps @ wt[:, :9].to(dtype).double().T + bias.double())
+    torch.testing.assert_close(y, ref64, atol=1e-5, rtol=1e-5)
+    fused = torch.empty_like(ref)
+    K.matmul_nt_bias_act(col, wd, fused, bias.to(DEV), relu=True)
+    assert torch.equal(fused, ref)
+    lin = torch.empty_like(ref)
+    K.matmul_nt_bias_act(col, wd, lin, bias.to(DEV), relu=False)
+    plain = torch.empty_like(ref)
+    K.matmul_nt(col, wd, plain)
+    K.bias_act(plain, bias.to(DEV), co, relu=False)
+    assert torch.equal(lin, plain)
+
+
+@pytest.mark.parametrize("dtype,c,c_out,b,h,w", [(torch.float32, 4, 6, 3, 13, 10), (torch.float32, 12, 130, 2, 9, 16),
+                                                 (torch.bfloat16, 8, 64, 5, 11, 11),
+                                                 (torch.bfloat16, 32, 130, 4, 16, 16)])
+def test_conv_gemm_equals_im2col_gemm(dtype, c, c_out, b, h, w):
+    """The implicit-GEMM convolution builds the same operand stages as im2col
+    and runs the same MFMA sequence: equal bit for bit; and within rounding of
+    float64 on the rounded operands."""
+    K = _kernels()
+    g = torch.Generator().manual_seed(c + c_out)
+    x = torch.relu(torch.randn(b, h, w, c, generator=g))
+    kp = K.padded_width(9 * c, K.MATCH_DTYPES[dtype])
+    wt = torch.zeros(c_out, kp)
+    wt[:, :9 * c] = torch.randn(c_out, 9 * c, generator=g) / math.sqrt(9 * c)
+    bias = 0.1 * torch.randn(c_out, generator=g)
+    wd, bd, xd = wt.to(dtype).to(DEV), bias.to(DEV), x.to(DEV)
+    print("conv_gemm", dtype, c, c_out, b, h, w, flush=True)
+    y = K.conv_gemm(xd, wd, bd)
+    col = torch.empty((b * ((h + 1) // 2) * ((w + 1) // 2), kp), dtype=dtype, device=DEV)
+    K.im2col(xd, col)
+    ref = torch.empty((col.shape[0], c_out), device=DEV)
+    K.matmul_nt_bias_act(col, wd, ref, bd, relu=True)
+    assert torch.equal(y, ref)
+    taps = torch.nn.functional.unfold(x.to(dtype).double().permute(0, 3, 1, 2), 3, padding=1, stride=2)
+    taps = taps.view(b, c, 9, -1).permute(0, 3, 2, 1).reshape(-1, 9 * c)
+    ref64 = torch.relu(taps @ wt[:, :9 * c].to(dtype).double().T + bias.double())
+    torch.testing.assert_close(y.cpu().double(), ref64, atol=1e-4, rtol=1e-4)
+
+
 def test_col2im_is_adjoint_of_im2col():
     K = _kernels()
     g = torch.Generator().manual_seed(1)
