@@ -482,30 +482,54 @@ def c5_workload(a, dev):
 
 
 def cpu_baseline(mp, rays, exp, dico, n_dict, seconds):
-    """The oracle (the reference's FP32 DI restated on torch CPU ops, same
-    chunking as the reference demo: 4096 x 4096) on 1,024 patterns against a
-    dictionary slice sized for ~`seconds` of work; patterns/s is scaled to
-    the full dictionary."""
+    """The oracle on the host cores, two variants of the reference's CPU DI
+    (same chunking as the reference demo: 4096 x 4096), each on 1,024
+    patterns against a dictionary slice sized for ~`seconds` of work, scaled
+    to the full dictionary: plain FP32 (`value`, the oracle's semantics) and
+    the reference's actual CPU default, int8 dynamic quantisation of each
+    dictionary chunk (quantized_if_x86, knn.py:52-72; `int8`)."""
+    import platform
+    import warnings
+
     import torch
 
     from oracle import di_oracle as O
 
+    warnings.filterwarnings("ignore", message=".*torch.ao.quantization is deprecated.*")
     cores = torch.get_num_threads()
-    n_sub = 4096
-    t = 0.0
-    while True:
-        t0 = time.perf_counter()
-        O.dictionary_index(mp, rays, exp, dico[:n_sub], top_k=16, dictionary_chunk_size=4096,
-                           experiment_chunk_size=4096)
-        t = time.perf_counter() - t0
-        if t >= seconds / 3 or n_sub >= n_dict:
-            break
-        n_sub = min(n_dict, int(n_sub * max(2.0, seconds / max(t, 1e-3))))
+
+    def timed(quantized, budget):
+        n_sub = 4096
+        while True:
+            t0 = time.perf_counter()
+            O.dictionary_index(mp, rays, exp, dico[:n_sub], top_k=16, dictionary_chunk_size=4096,
+                               experiment_chunk_size=4096, quantized=quantized)
+            t = time.perf_counter() - t0
+            if t >= budget / 3 or n_sub >= n_dict:
+                return n_sub, t
+            n_sub = min(n_dict, int(n_sub * max(2.0, budget / max(t, 1e-3))))
+
+    n_sub, t = timed(False, seconds)
     rate = exp.shape[0] / (t * n_dict / n_sub)
     log(f"cpu baseline: {exp.shape[0]} patterns x {n_sub} dictionary entries in {t:.1f}s on {cores} threads")
+    n8, t8 = timed(True, seconds / 2)
+    rate8 = exp.shape[0] / (t8 * n_dict / n8)
+    log(f"cpu baseline int8: {exp.shape[0]} patterns x {n8} dictionary entries in {t8:.1f}s")
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    cfg = torch.__config__.show()
+    libs = [l.strip(" -") for l in cfg.splitlines() if "Math Kernel Library" in l or "MKL-DNN" in l]
     return {"value": round(rate, 3), "unit": "patterns/s", "cores": cores, "kind": "port",
             "sample": f"{exp.shape[0]} patterns x first {n_sub} of {n_dict} dictionary entries "
-                      f"({t:.1f} s), scaled to the full dictionary"}
+                      f"({t:.1f} s), scaled to the full dictionary",
+            "int8": {"value": round(rate8, 3), "unit": "patterns/s",
+                     "what": "the reference's CPU default: int8 quantize_dynamic of each dictionary chunk",
+                     "sample": f"{exp.shape[0]} patterns x first {n8} dictionary entries ({t8:.1f} s), scaled"},
+            "host": {"os_cpu_count": os.cpu_count(), "threads_used": cores, "model": model, "libraries": libs}}
 
 
 if __name__ == "__main__":

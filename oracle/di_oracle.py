@@ -480,11 +480,48 @@ def project_dictionary(mp: torch.Tensor, rays: torch.Tensor, quats: torch.Tensor
 # --------------------------------------------------------------------------
 
 
-class ChunkedKNN:
-    """utils/knn.py:106-287 with match_dtype=float32, quantized_via_ao=False."""
+class _NoBiasLinear(torch.nn.Module):
+    """The bias-free Linear layer holding a dictionary chunk as its weight
+    (knn.py:36-49)."""
 
-    def __init__(self, data_size: int, query_size: int, topk: int, match_dtype=torch.float32):
+    def __init__(self, weight: torch.Tensor):
+        super().__init__()
+        self.fc = torch.nn.Linear(weight.shape[1], weight.shape[0], bias=False)
+        self.fc.weight.data = weight
+
+    def forward(self, x):
+        return self.fc(x)
+
+
+def quant_model_data(data: torch.Tensor) -> torch.nn.Module:
+    """knn.py:52-58: the chunk as the weight of a Linear, dynamically
+    quantised to int8 (per-tensor weight scale; activations quantised per call)
+    by torch's quantize_dynamic (the library call the reference makes)."""
+    from torch.ao.quantization import quantize_dynamic
+
+    layer = _NoBiasLinear(data)
+    layer.eval()
+    return quantize_dynamic(model=layer, qconfig_spec={torch.nn.Linear}, dtype=torch.qint8, inplace=True)
+
+
+def topk_quantized_data(query_batch: torch.Tensor, quantized_data: torch.nn.Module, k: int):
+    """knn.py:70-72."""
+    res = torch.topk(quantized_data(query_batch), k, dim=1, largest=True, sorted=False)
+    return res.values, res.indices
+
+
+class ChunkedKNN:
+    """utils/knn.py:106-287, angular metric. match_dtype float32; quantized
+    selects the reference's CPU default (quantized_via_ao=True on a CPU
+    device: int8 dynamic quantisation of each data chunk, knn.py:135-146,
+    :177-180, :196-199, :236-239)."""
+
+    def __init__(self, data_size: int, query_size: int, topk: int, match_dtype=torch.float32,
+                 quantized: bool = False):
         self.data_size, self.query_size, self.topk = data_size, query_size, topk
+        self.quantized = bool(quantized)
+        if self.quantized:
+            match_dtype = torch.float32  # knn.py:135-143
         self.match_dtype = match_dtype
         ind_dtype = torch.int64 if data_size < 2 ** 31 else torch.int32
         self.knn_indices = torch.empty((query_size, topk), dtype=ind_dtype)
@@ -497,7 +534,12 @@ class ChunkedKNN:
         data_chunk = data_chunk.to(self.match_dtype)
         self.curr_end += data_chunk.shape[0]
         self.curr_start = self.curr_end - data_chunk.shape[0]
-        self.prepared_data = data_chunk
+        self.prepared_data = quant_model_data(data_chunk) if self.quantized else data_chunk
+
+    def _search(self, query):
+        if self.quantized:
+            return topk_quantized_data(query, self.prepared_data, self.topk)
+        return self._knn_batch(self.prepared_data, query, self.topk)
 
     @staticmethod
     def _knn_batch(data, query, topk):
@@ -509,7 +551,7 @@ class ChunkedKNN:
         """knn.py:182-217."""
         if self.prepared_data is None:
             raise ValueError("Data chunk is not set.")
-        d, i = self._knn_batch(self.prepared_data, query.to(self.match_dtype), self.topk)
+        d, i = self._search(query.to(self.match_dtype))
         i = i + self.curr_start
         md = torch.cat((self.knn_distances, d), dim=1)
         mi = torch.cat((self.knn_indices, i), dim=1)
@@ -522,7 +564,7 @@ class ChunkedKNN:
         if self.prepared_data is None:
             raise ValueError("Data chunk is not set.")
         query_end = query_start + query_chunk.shape[0]
-        d, i = self._knn_batch(self.prepared_data, query_chunk.to(self.match_dtype), self.topk)
+        d, i = self._search(query_chunk.to(self.match_dtype))
         i = i + self.curr_start
         md = torch.cat((self.knn_distances[query_start:query_end], d), dim=1)
         mi = torch.cat((self.knn_indices[query_start:query_end], i), dim=1)
@@ -538,13 +580,16 @@ class ChunkedKNN:
 
 def dictionary_index(mp: torch.Tensor, rays: torch.Tensor, exp_pats: torch.Tensor, dictionary: torch.Tensor,
                      top_k: int = 1, dictionary_chunk_size: int = 256, experiment_chunk_size: int = 256,
-                     signal_mask: Optional[torch.Tensor] = None, match_dtype=torch.float32):
+                     signal_mask: Optional[torch.Tensor] = None, match_dtype=torch.float32,
+                     quantized: bool = False):
     """dictionary_index_orientations (indexing.py:14-211) for one phase, with
     the dictionary passed in (the reference draws it with an unseeded
-    randperm) and plain FP32 matching. exp_pats: (N, H*W) float32.
+    randperm); plain FP32 matching, or with quantized=True the reference's
+    CPU default (quantized_if_x86, int8 quantize_dynamic of each dictionary
+    chunk). exp_pats: (N, H*W) float32.
     Returns (indices (N, k) int64, scores (N, k))."""
     n_exp = exp_pats.shape[0]
-    knn = ChunkedKNN(len(dictionary), n_exp, top_k, match_dtype)
+    knn = ChunkedKNN(len(dictionary), n_exp, top_k, match_dtype, quantized=quantized)
     for ori_batch in torch.split(dictionary, dictionary_chunk_size):
         rot = qu_apply(ori_batch[:, None, :], rays[None, ...])
         sim = interpolate(mp, rot, normalize_coords=False).view(len(ori_batch), -1)

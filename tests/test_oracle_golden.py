@@ -239,6 +239,33 @@ def test_chunked_knn_equals_float64_brute_force_and_is_chunk_invariant():
         assert torch.allclose(val.double(), ref_val, atol=1e-4)
 
 
+def test_chunked_knn_int8_path_finds_planted_matches_within_quantisation_error():
+    """The reference's CPU default (int8 quantize_dynamic of each data chunk,
+    knn.py:52-72): queries that are noisy copies of data rows find their
+    source row, global indices survive the chunked merge, and every reported
+    score is the float score of that pair up to the int8 steps (weights: the
+    chunk's max|w| / 127; activations: the query row range / 255)."""
+    g = torch.Generator().manual_seed(12)
+    data = torch.randn(300, 64, generator=g)
+    src = torch.randperm(300, generator=g)[:40]
+    query = data[src] + 0.05 * torch.randn(40, 64, generator=g)
+    kn = O.ChunkedKNN(300, 40, 3, quantized=True)
+    for s in range(0, 300, 128):
+        kn.set_data_chunk(data[s:s + 128])
+        for qs in range(0, 40, 16):
+            kn.query_chunk(query[qs:qs + 16], qs)
+    idx, val = kn.retrieve_topk()
+    assert torch.equal(idx[:, 0], src)
+    assert bool((val[:, :-1] >= val[:, 1:]).all())
+    exact = (query.double() @ data.double().T).gather(1, idx)
+    step_w = float(data.abs().max()) / 127
+    step_x = float((query.max() - query.min())) / 255
+    bound = 64 * (float(query.abs().max()) * step_w + float(data.abs().max()) * step_x + step_w * step_x)
+    assert float((val.double() - exact).abs().max()) < bound
+    # the quantised scores are not the float ones (the int8 path really ran)
+    assert float((val.double() - exact).abs().max()) > 1e-4
+
+
 def test_query_before_data_raises():
     kn = O.ChunkedKNN(10, 4, 1)
     with pytest.raises(ValueError):
