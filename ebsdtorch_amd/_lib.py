@@ -108,6 +108,10 @@ _SIGNATURES = {
     "ebsd_gemm_nt_bias_act": (
         _c_int, [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int,
                  _c_void_p]),
+    "ebsd_grid_refine_step": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_void_p,
+                 _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p,
+                 _c_void_p]),
     "ebsd_nn_col2im": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p]),
     "ebsd_nn_bias_act": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p]),
     "ebsd_nn_relu_bwd": (_c_int, [_c_void_p, _c_void_p, ctypes.c_longlong, _c_void_p]),

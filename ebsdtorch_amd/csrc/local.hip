@@ -87,9 +87,40 @@ __device__ __forceinline__ float4 load_quad<ushort4>(const ushort4* __restrict__
                      __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
 }
 
+// Source-index affine maps of the side-by-side Lambert coordinates (u, v) in
+// [-1, 1]^2 of one hemisphere: ix = u qx + (x0, or xs in the southern
+// hemisphere), iy = v qy + y0.
+struct LambertAffine {
+  float qx, qy, x0, xs, y0;
+};
+
+// grid_sample with align_corners=False on an (H, W) side-by-side image:
+// ix = u W/4 + W/4 - 1/2 (+ W/2 south), iy = v H/2 + H/2 - 1/2.
+__device__ __forceinline__ LambertAffine affine_unaligned(int H, int W) {
+  LambertAffine a;
+  a.qx = 0.25f * (float)W;
+  a.qy = 0.5f * (float)H;
+  a.x0 = a.qx - 0.5f;
+  a.xs = a.x0 + 0.5f * (float)W;
+  a.y0 = a.qy - 0.5f;
+  return a;
+}
+
+// align_corners=True (master_pattern.py:197-206 as called by refining.py:200):
+// ix = (u/4 + 1/4 (+ 1/2 south)) (W - 1), iy = (v/2 + 1/2) (H - 1).
+__device__ __forceinline__ LambertAffine affine_aligned(int H, int W) {
+  LambertAffine a;
+  a.qx = 0.25f * (float)(W - 1);
+  a.qy = 0.5f * (float)(H - 1);
+  a.x0 = a.qx;
+  a.xs = a.qx + 0.5f * (float)(W - 1);
+  a.y0 = a.qy;
+  return a;
+}
+
 template <typename QuadT>
-__device__ __forceinline__ float sample_fast(const QuadT* __restrict__ quads, int H, int W, float qx, float qy,
-                                             float x_hemi, float vx, float vy, float vz) {
+__device__ __forceinline__ float sample_fast(const QuadT* __restrict__ quads, int H, int W, const LambertAffine& af,
+                                             float vx, float vy, float vz) {
   const float ax = fabsf(vx), ay = fabsf(vy);
   const float factor = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-2.0f, fabsf(vz), 2.0f), 0.0f));
   const bool cond = ay <= ax;
@@ -103,8 +134,8 @@ __device__ __forceinline__ float sample_fast(const QuadT* __restrict__ quads, in
   const float simpler = sf * 0.70710678118654752440f;                    // 2 / sqrt(8)
   const float u = cond ? simpler : arct;
   const float v = cond ? arct : simpler;
-  float ix = fmaf(u, qx, (vz < 0.0f) ? x_hemi + 0.5f * (float)W : x_hemi);
-  float iy = fmaf(v, qy, qy - 0.5f);
+  float ix = fmaf(u, af.qx, (vz < 0.0f) ? af.xs : af.x0);
+  float iy = fmaf(v, af.qy, af.y0);
   ix = __builtin_amdgcn_fmed3f(ix, 0.0f, (float)(W - 1));
   iy = __builtin_amdgcn_fmed3f(iy, 0.0f, (float)(H - 1));
   const float x0f = floorf(ix);
@@ -120,13 +151,19 @@ __device__ __forceinline__ float sample_fast(const QuadT* __restrict__ quads, in
 }
 
 template <typename QuadT>
-__device__ __forceinline__ float sample_candidate(const QuadT* __restrict__ quads, int H, int W, const Rot& R,
-                                                  float rx, float ry, float rz) {
+__device__ __forceinline__ float sample_rotated(const QuadT* __restrict__ quads, int H, int W,
+                                                const LambertAffine& af, const Rot& R, float rx, float ry,
+                                                float rz) {
   const float vx = fmaf(R.m[0], rx, fmaf(R.m[1], ry, R.m[2] * rz));
   const float vy = fmaf(R.m[3], rx, fmaf(R.m[4], ry, R.m[5] * rz));
   const float vz = fmaf(R.m[6], rx, fmaf(R.m[7], ry, R.m[8] * rz));
-  const float qx = 0.25f * (float)W, qy = 0.5f * (float)H;
-  return sample_fast(quads, H, W, qx, qy, qx - 0.5f, vx, vy, vz);
+  return sample_fast(quads, H, W, af, vx, vy, vz);
+}
+
+template <typename QuadT>
+__device__ __forceinline__ float sample_candidate(const QuadT* __restrict__ quads, int H, int W, const Rot& R,
+                                                  float rx, float ry, float rz) {
+  return sample_rotated(quads, H, W, affine_unaligned(H, W), R, rx, ry, rz);
 }
 
 template <typename QuadT>
@@ -322,6 +359,117 @@ __global__ __launch_bounds__(kLocalWaves * 64) void local_rescore_kernel(
   }
 }
 
+// q = standardised (w >= 0) Hamilton product a * b (quaternions.py:78-91).
+__device__ __forceinline__ void qu_prod_std(const float* a, const float* b, float* q) {
+  float w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  float x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+  float y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+  float z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+  const float s = w < 0.0f ? -1.0f : 1.0f;
+  q[0] = s * w;
+  q[1] = s * x;
+  q[2] = s * y;
+  q[3] = s * z;
+}
+
+// One iteration of the grid refinement of refining.py:172-237 for query
+// pattern i: the candidates qu_std(g_j * c_i) (g_j the local grid at the
+// pattern's shrink level), each moved to the fundamental zone as
+// ori_to_fz_laue does (laue_fz_ori.py:35-81: the right product with the
+// symmetry operator giving the largest |w|, first such operator), projected
+// through the align_corners=True lookup of the (binned) master pattern and
+// scored by the dot product of the zero-mean candidate with the zero-mean
+// query (not normalised). Writes the best (score, j, reduced quaternion),
+// ties to the lower j (torch.max). Layout of local_rescore_kernel: one wave
+// per query, lane = slice * kp + c; candidates in chunks of kp.
+template <typename QuadT>
+__global__ __launch_bounds__(kLocalWaves * 64) void grid_refine_kernel(
+    const QuadT* __restrict__ quads, int H, int W, const float* __restrict__ rays, int n_pix,
+    const float* __restrict__ Q, int ld, int m, const float* __restrict__ centers, const float* __restrict__ grids,
+    int n_grid, int n_levels, const int* __restrict__ level, const float* __restrict__ ops, int n_ops, int kp,
+    float* __restrict__ best_val, int* __restrict__ best_idx, float* __restrict__ best_q) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kLocalWaves + (threadIdx.x >> 6);
+  if (i >= m) return;  // whole waves only; the kernel has no block barrier
+  const int c = lane & (kp - 1);
+  const int n_slices = 64 / kp;
+  const int slice = lane / kp;
+  const float* __restrict__ xrow = Q + (size_t)i * ld;
+  // levels outside [0, n_levels) are clamped (the grids buffer has n_levels)
+  const float* __restrict__ grid = grids + (size_t)min(max(level[i], 0), n_levels - 1) * n_grid * 4;
+  float ctr[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) ctr[k] = centers[(size_t)i * 4 + k];
+  const LambertAffine af = affine_aligned(H, W);
+  const float inv_d = 1.0f / (float)n_pix;
+  const float ident[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+
+  float bv = -INFINITY;
+  int bj = INT_MAX;
+  float bq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int j0 = 0; j0 < n_grid; j0 += kp) {
+    const int j = j0 + c;
+    const bool valid = j < n_grid;
+    float q[4], qf[4];
+    qu_prod_std(grid + (size_t)min(j, n_grid - 1) * 4, ctr, q);
+    int og = 0;
+    float amax = -1.0f;
+    for (int o = 0; o < n_ops; ++o) {
+      const float* g = ops + 4 * o;
+      const float a = fabsf(q[0] * g[0] - q[1] * g[1] - q[2] * g[2] - q[3] * g[3]);
+      if (a > amax) {
+        amax = a;
+        og = o;
+      }
+    }
+    qu_prod_std(q, ops + 4 * og, qf);
+    const Rot R = compose_matrix(ident, qf);
+    const float p0 = sample_rotated(quads, H, W, af, R, rays[0], rays[1], rays[2]);
+    float sx = 0.0f, sp = 0.0f, sxp = 0.0f;
+    for (int d = slice; d < n_pix; d += n_slices) {
+      const float* r = rays + 3 * d;
+      const float xv = xrow[d];
+      const float p = sample_rotated(quads, H, W, af, R, r[0], r[1], r[2]) - p0;
+      sx += xv;
+      sp += p;
+      sxp = fmaf(xv, p, sxp);
+    }
+    for (int off = kp; off < 64; off <<= 1) {
+      sx += __shfl_xor(sx, off, 64);
+      sp += __shfl_xor(sp, off, 64);
+      sxp += __shfl_xor(sxp, off, 64);
+    }
+    // sum (x - mean x)(p - mean p) = Sxp - Sp Sx / D, unchanged by the p0 shift
+    const float score = valid ? sxp - sp * sx * inv_d : -INFINITY;
+    if (valid && topk_better(score, j, bv, bj)) {
+      bv = score;
+      bj = j;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) bq[k] = qf[k];
+    }
+  }
+  for (int off = 1; off < kp; off <<= 1) {
+    const float ov = __shfl_xor(bv, off, 64);
+    const int oj = __shfl_xor(bj, off, 64);
+    float oq[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) oq[k] = __shfl_xor(bq[k], off, 64);
+    if (topk_better(ov, oj, bv, bj)) {
+      bv = ov;
+      bj = oj;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) bq[k] = oq[k];
+    }
+  }
+  if (lane == 0) {
+    const bool found = bj != INT_MAX;
+    best_val[i] = found ? bv : -INFINITY;
+    best_idx[i] = found ? bj : -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) best_q[(size_t)i * 4 + k] = found ? bq[k] : ctr[k];
+  }
+}
+
 }  // namespace
 
 extern "C" int ebsd_mp_quad_table(const float* mp, int mp_rows, int mp_cols, void* quads, int dtype,
@@ -389,6 +537,36 @@ extern "C" int ebsd_local_rescore(const void* quads, int quad_dtype, int mp_rows
     hipLaunchKernelGGL(local_rescore_kernel<ushort4>, g, b, 0, st, static_cast<const ushort4*>(quads), mp_rows,
                        mp_cols, rays, n_pix, q, ld, m, centers, grid, n_grid, cand, k, kp, out_val, best_val,
                        best_idx);
+  else
+    return EBSD_ERR_DTYPE;
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_grid_refine_step(const void* quads, int quad_dtype, int mp_rows, int mp_cols, const float* rays,
+                                     int n_pix, const float* q, int m, int ld, const float* centers,
+                                     const float* grids, int n_grid, int n_levels, const int* level,
+                                     const float* ops, int n_ops, float* best_val, int* best_idx, float* best_q,
+                                     void* stream) {
+  if (!quads || !rays || !q || !centers || !grids || !level || !ops || !best_val || !best_idx || !best_q)
+    return EBSD_ERR_NULL;
+  if (mp_rows < 2 || mp_cols < 2 || (long long)mp_rows * mp_cols >= (1LL << 24)) return EBSD_ERR_SHAPE;
+  if (n_pix < 1 || ld < n_pix || m < 0 || n_grid < 1 || n_levels < 1 || n_ops < 1 || n_ops > 64)
+    return EBSD_ERR_SHAPE;
+  if ((reinterpret_cast<uintptr_t>(quads) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (m == 0) return EBSD_OK;
+  int kp = 1;
+  while (kp < n_grid && kp < 64) kp <<= 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g((unsigned)((m + kLocalWaves - 1) / kLocalWaves)), b(kLocalWaves * 64);
+  if (quad_dtype == EBSD_DTYPE_F32)
+    hipLaunchKernelGGL(grid_refine_kernel<float4>, g, b, 0, st, static_cast<const float4*>(quads), mp_rows, mp_cols,
+                       rays, n_pix, q, ld, m, centers, grids, n_grid, n_levels, level, ops, n_ops, kp, best_val,
+                       best_idx, best_q);
+  else if (quad_dtype == EBSD_DTYPE_F16)
+    hipLaunchKernelGGL(grid_refine_kernel<ushort4>, g, b, 0, st, static_cast<const ushort4*>(quads), mp_rows,
+                       mp_cols, rays, n_pix, q, ld, m, centers, grids, n_grid, n_levels, level, ops, n_ops, kp,
+                       best_val, best_idx, best_q);
   else
     return EBSD_ERR_DTYPE;
   EBSD_LAUNCH_CHECK();

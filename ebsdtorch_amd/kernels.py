@@ -646,6 +646,45 @@ def local_rescore(quads: torch.Tensor, rays: torch.Tensor, queries: PatternMatri
     return best_val, best_idx, scores
 
 
+def grid_refine_step(quads: torch.Tensor, rays: torch.Tensor, queries: PatternMatrix, centers: torch.Tensor,
+                     grids: torch.Tensor, level: torch.Tensor, ops: torch.Tensor):
+    """One grid-refinement iteration (ebsd_grid_refine_step). quads: quad
+    table of the (binned) master pattern; queries: zero-mean FP32 rows;
+    grids (L, G, 4) local rotations per shrink level; level (N,) int32;
+    ops (n_ops, 4) symmetry operators. Returns (best dot (N,) f32, grid index
+    (N,) int32, fundamental-zone quaternion (N, 4) f32)."""
+    if queries.dtype != DTYPE_F32:
+        raise ValueError("grid refinement scores FP32 queries")
+    require_device(quads, "quads")
+    qdt = {torch.float32: DTYPE_F32, torch.float16: DTYPE_F16}.get(quads.dtype)
+    if qdt is None or quads.dim() != 3 or quads.shape[2] != 4 or not quads.is_contiguous():
+        raise ValueError("quads must be the (S, C, 4) table of mp_quad_table")
+    rays = _f32_on_device(rays, "rays")
+    centers = _f32_on_device(centers, "centers")
+    grids = _f32_on_device(grids, "grids")
+    ops = _f32_on_device(ops, "ops")
+    n = queries.rows
+    if rays.shape != (queries.n_pix, 3) or centers.shape != (n, 4):
+        raise ValueError("rays (D, 3) and centers (N, 4) must match the queries")
+    if grids.dim() != 3 or grids.shape[2] != 4 or ops.dim() != 2 or ops.shape[1] != 4:
+        raise ValueError("grids must be (L, G, 4) and ops (n_ops, 4)")
+    require_device(level, "level")
+    level = level.to(torch.int32).contiguous()
+    if level.shape != (n,):
+        raise ValueError("level must be (N,)")
+    dev = centers.device
+    best_val = torch.empty(n, dtype=torch.float32, device=dev)
+    best_idx = torch.empty(n, dtype=torch.int32, device=dev)
+    best_q = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    st = _lib.load().ebsd_grid_refine_step(quads.data_ptr(), qdt, quads.shape[0], quads.shape[1], rays.data_ptr(),
+                                           queries.n_pix, queries.data.data_ptr(), n, queries.ld, centers.data_ptr(),
+                                           grids.data_ptr(), grids.shape[1], grids.shape[0], level.data_ptr(),
+                                           ops.data_ptr(), ops.shape[0], best_val.data_ptr(), best_idx.data_ptr(),
+                                           best_q.data_ptr(), _lib.stream_ptr(dev))
+    check(st, "ebsd_grid_refine_step")
+    return best_val, best_idx, best_q
+
+
 def rot6d_to_quat(z6: torch.Tensor) -> torch.Tensor:
     """(B, >= 6) network outputs -> (B, 4) unit quaternions, w >= 0, of the
     Gram-Schmidt rotation the geodesic loss scores."""

@@ -357,6 +357,33 @@ int ebsd_local_rescore(const void* quads, int quad_dtype, int mp_rows, int mp_co
                        const int* cand, int k, float* out_val, float* best_val,
                        int* best_idx, void* stream);
 
+/*
+ * One iteration of orientation grid refinement for m patterns: for pattern
+ * i and each local rotation g_j of its level's grid
+ * (grids[level[i]] (n_grid, 4) f32, the cubochoric local grid shrunk
+ * level[i] times, as unit quaternions), the candidate qu_std(g_j * c_i) is
+ * moved to the fundamental zone (right product with the first of the n_ops
+ * symmetry operators `ops` (n_ops, 4) f32 that maximises |w|, then w >= 0),
+ * projected through the bilinear, border, align_corners=True lookup of the
+ * master pattern (its quad table, mp_rows x mp_cols, F32 or F16; the
+ * virtually binned master pattern when the patterns are binned) along
+ * `rays` (n_pix, 3) f32, and scored by the dot product of the zero-mean
+ * candidate with the pattern row q (zero-mean, not normalised). Writes the
+ * best score best_val (m,), its grid index best_idx (m,) (ties: lower j) and
+ * its reduced quaternion best_q (m, 4).
+ *   q (m, ld) f32; centers (m, 4) f32; level (m,) int32 in [0, n_levels)
+ *   (clamped to that range)
+ * Replaces: the body of the iteration loop of refining.py:172-237
+ * (orientation_grid_refinement: cu2qu of the grid is done once per level by
+ * the caller; qu_prod, ori_to_fz_laue, qu_apply, MasterPattern.interpolate
+ * with align_corners=True, the einsum dot products and torch.max fused).
+ */
+int ebsd_grid_refine_step(const void* quads, int quad_dtype, int mp_rows, int mp_cols,
+                          const float* rays, int n_pix, const float* q, int m, int ld,
+                          const float* centers, const float* grids, int n_grid,
+                          int n_levels, const int* level, const float* ops, int n_ops,
+                          float* best_val, int* best_idx, float* best_q, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

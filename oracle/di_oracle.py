@@ -684,3 +684,91 @@ def pca_dictionary_index(mp, coords, exp_pats, learn_dictionary, index_dictionar
             q = q - torch.mean(q, dim=-1, keepdim=True)
             knn.query_chunk(torch.matmul(q, pca), query_start=int(idx[0]))
     return knn.retrieve_topk()
+
+
+# --------------------------------------------------------------------------
+# Orientation grid refinement (ebsd/refining.py:13-244), one phase, average
+# pattern centre
+# --------------------------------------------------------------------------
+
+
+def get_local_orientation_grid(semi_edge_in_degrees: float, kernel_radius_in_steps: int,
+                               axial_grid_dimension: int = 3) -> torch.Tensor:
+    """refining.py:13-68 (cubochoric coordinates, float32)."""
+    semi = semi_edge_in_degrees * torch.pi ** (2 / 3) / 360.0
+    cu = torch.linspace(-semi, semi, 2 * kernel_radius_in_steps + 1)
+    cu = torch.stack(torch.meshgrid(cu, cu, cu, indexing="ij"), dim=-1).reshape(-1, 3)
+    if axial_grid_dimension == 2:
+        cu = cu[torch.any(cu == 0, dim=-1)]
+    elif axial_grid_dimension == 1:
+        cu = cu[torch.sum(cu == 0, dim=-1) >= 2]
+    elif axial_grid_dimension != 3:
+        raise ValueError("axial_grid_dimension must be 1, 2, or 3.")
+    return cu
+
+
+def ori_to_fz_laue(quats: torch.Tensor, laue_id: int) -> torch.Tensor:
+    """laue_fz_ori.py:35-81: the right product with the symmetry operator
+    that maximises |w| (qu_prod_pos_real, argmax: first maximum)."""
+    shape = quats.shape
+    ops = laue_elements(laue_id).to(quats.dtype)
+    q = quats.reshape(-1, 1, 4)
+    w = (q[..., 0] * ops[:, 0] - q[..., 1] * ops[:, 1] - q[..., 2] * ops[:, 2] - q[..., 3] * ops[:, 3]).abs()
+    best = torch.argmax(w, dim=-1)
+    return qu_prod(quats.reshape(-1, 4), ops[best]).reshape(shape)
+
+
+def interpolate_aligned(mp: torch.Tensor, coords_3d: torch.Tensor) -> torch.Tensor:
+    """MasterPattern.interpolate with mode bilinear, padding border,
+    align_corners=True, normalize_coords=False (master_pattern.py:141-209, as
+    refining.py:198-204 calls it)."""
+    g = rosca_lambert_side_by_side(coords_3d)
+    n = g.numel() // 2
+    out = F.grid_sample(mp[None, None], g.reshape(1, 1, n, 2), mode="bilinear",
+                        padding_mode="border", align_corners=True)
+    return out.view(coords_3d.shape[:-1])
+
+
+def orientation_grid_refinement(mp: torch.Tensor, laue_id: int, rays: torch.Tensor, exp_pats: torch.Tensor,
+                                orientations: torch.Tensor, grid_semi_edge_in_degrees: float, batch_size: int,
+                                virtual_binning: int = 1, n_iter: int = 3, axial_grid_dimension: int = 3,
+                                kernel_radius_in_steps: int = 1, shrink_factor: float = 0.5,
+                                match_dtype=torch.float32):
+    """orientation_grid_refinement (refining.py:71-244) for one phase with
+    an average pattern centre. mp (S, 2S) unbinned master pattern; rays
+    (D_b, 3) unit sample-frame directions of the binned detector
+    (get_coords_sample_frame(binning=(b, b)) normalised, refining.py:112-129);
+    exp_pats (N, H, W) unbinned patterns; orientations (N, 4).
+    Returns (refined orientations (N, 4), dots (N, n_iter))."""
+    mp_b = mp
+    if virtual_binning > 1:
+        # master_pattern.py:94-113 (MasterPattern.bin)
+        mp_b = F.avg_pool2d(mp[None, None], virtual_binning, stride=virtual_binning)[0, 0]
+    n = exp_pats.shape[0]
+    cu_grid = get_local_orientation_grid(grid_semi_edge_in_degrees, kernel_radius_in_steps,
+                                         axial_grid_dimension).view(1, -1, 3)
+    dots = torch.zeros((n, n_iter))
+    out = orientations.clone()
+    for idx in torch.split(torch.arange(n), batch_size):
+        cu_cur = cu_grid.clone().repeat(len(idx), 1, 1)
+        pats = exp_pats[idx]
+        if virtual_binning > 1:
+            # experiment_pats.py:342
+            pats = F.avg_pool2d(pats[:, None], virtual_binning)[:, 0]
+        pats = pats.reshape(len(idx), -1)
+        pats = pats - torch.mean(pats, dim=-1, keepdim=True)
+        q_cur = out[idx]
+        for j in range(n_iter):
+            q_grid = cu2qu(cu_cur)
+            q_aug = qu_prod(q_grid, q_cur[:, None, :])
+            q_aug = ori_to_fz_laue(q_aug, laue_id)
+            rot = qu_apply(q_aug[:, :, None, :], rays[None, None, :, :])
+            sim = interpolate_aligned(mp_b, rot).view(len(idx), q_aug.shape[1], -1)
+            sim = sim - torch.mean(sim, dim=-1, keepdim=True)
+            d = torch.einsum("ij,ikj->ik", pats.to(match_dtype), sim.to(match_dtype))
+            best_d, best_i = torch.max(d, dim=-1)
+            dots[idx, j] = best_d.float()
+            q_cur = q_aug[torch.arange(len(idx)), best_i]
+            cu_cur[best_i == ((cu_cur.shape[1] - 1) / 2)] *= shrink_factor
+        out[idx] = q_cur
+    return out, dots
