@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--agree-sample", type=int, default=16384,
                    help="patterns re-indexed in plain FP32 to measure top-1 agreement")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
+    p.add_argument("--refine", type=int, default=1, help="measure grid refinement of the indexed batch (0 = skip)")
     p.add_argument("--learned-train", type=int, default=2_000_000,
                    help="training patterns of the learned-proposal form (0 = skip that form)")
     p.add_argument("--learned-batch", type=int, default=65536, help="patterns indexed by the learned form")
@@ -181,6 +182,10 @@ def main():
     proj_bytes = n_dict * (D * out_bytes + 16) + 2 * a.mp_size * a.mp_size * 4 + 12 * D
     proj_gbps = proj_bytes / (proj_kernel_ms / 1000) / 1e9
 
+    refinement = None
+    if a.refine:
+        refinement = refinement_form(a, mp, geom, exp, ori, eng.orientations[idx[:, 0].long()], det, dev)
+
     learned = None
     if a.learned_train > 0:
         del eng
@@ -244,10 +249,92 @@ def main():
             "achieved": round(proj_gbps, 1), "peak": PEAKS["hbm_GBps"], "unit": "GB/s",
             "frac": round(proj_gbps / PEAKS["hbm_GBps"], 4), "bytes": proj_bytes,
         },
+        "refinement": refinement,
         "learned_proposal": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def refinement_form(a, mp, geom, exp, truth, found, det, dev):
+    """Orientation grid refinement of the indexed batch (refining.py:71-244
+    with its defaults: kernel radius 1 = 27 candidates, 3 iterations, shrink
+    0.5; semi-edge 1 degree, no binning, match in FP32): patterns/s of the
+    whole call with the FP8-indexed orientations as the start, disorientation
+    to the truth before and after, the refinement kernel alone, and the
+    oracle on a CPU sample."""
+    import torch
+
+    from ebsdtorch_amd.experiment_pats import ExperimentPatterns
+    from ebsdtorch_amd.master_pattern import MasterPattern
+    from ebsdtorch_amd.orientations import disori_angle_laue
+    from ebsdtorch_amd.refining import orientation_grid_refinement
+
+    n = exp.shape[0]
+    pats = exp.view(n, det, det)
+    mpo = MasterPattern(mp, laue_group=11)
+    kw = dict(grid_semi_edge_in_degrees=1.0, batch_size=n, virtual_binning=1, n_iter=3, kernel_radius_in_steps=1)
+
+    def run(q0):
+        e = ExperimentPatterns(pats[:q0.shape[0]])
+        e.set_orientations(q0.clone())
+        e.phases = torch.zeros(q0.shape[0], dtype=torch.uint8, device=dev)
+        return e, orientation_grid_refinement(mpo, geom, e, **kw)[0]
+
+    run(found[:4096].contiguous())  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e, dots = run(found)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    before = torch.rad2deg(disori_angle_laue(found.double(), truth.double(), 11))
+    after = torch.rad2deg(disori_angle_laue(e.get_orientations().double(), truth.double(), 11))
+    qs = torch.tensor([0.5, 0.9, 0.99], dtype=torch.float64, device=dev)
+    # one kernel launch over the batch (first iteration)
+    from ebsdtorch_amd import kernels
+    from ebsdtorch_amd.refining import _grid_levels
+    from ebsdtorch_amd.orientations import laue_elements
+
+    rays = geom.get_coords_sample_frame(binning=(1, 1)).view(-1, 3)
+    rays = (rays / rays.norm(dim=-1, keepdim=True)).to(dev).contiguous()
+    grids = _grid_levels(1.0, 1, 3, 0.5, 3).to(dev)
+    x = kernels.prep_patterns(exp, 0, zero_mean=True, unit_norm=False)
+    quads = kernels.mp_quad_table(mp, torch.float32)
+    ops = laue_elements(11).to(dev, torch.float32)
+    lev = torch.zeros(n, dtype=torch.int32, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    kernels.grid_refine_step(quads, rays, x, found, grids, lev, ops)
+    e1.record()
+    torch.cuda.synchronize()
+    step_ms = e0.elapsed_time(e1)
+    samples = n * grids.shape[1] * rays.shape[0]
+    cpu = None
+    if a.cpu_seconds > 0:
+        from oracle import di_oracle as O
+
+        ns = 256
+        t1 = time.perf_counter()
+        O.orientation_grid_refinement(mp.cpu(), 11, rays.cpu(), pats[:ns].cpu(), found[:ns].cpu(), 1.0, ns,
+                                      n_iter=3, kernel_radius_in_steps=1)
+        tc = time.perf_counter() - t1
+        cpu = {"patterns_per_s": round(ns / tc, 2), "cores": torch.get_num_threads(), "kind": "port",
+               "sample": f"{ns} patterns ({tc:.1f} s)"}
+    out = {
+        "workload": f"grid refinement of {n} FP8-indexed patterns {det}x{det}: 27 candidates (radius 1, semi-edge "
+                    f"1 deg), 3 iterations, shrink 0.5",
+        "patterns_per_s": round(n / wall, 1),
+        "ms": round(wall * 1000, 2),
+        "disorientation_deg_quantiles_before": [round(v, 3) for v in torch.quantile(before, qs).tolist()],
+        "disorientation_deg_quantiles_after": [round(v, 3) for v in torch.quantile(after, qs).tolist()],
+        "mean_best_dot_by_iteration": [round(v, 4) for v in dots.mean(0).tolist()],
+        "step_kernel": {"ms": round(step_ms, 3), "samples": samples,
+                        "gsamples_per_s": round(samples / (step_ms / 1000) / 1e9, 1)},
+        "cpu_oracle": cpu,
+    }
+    log(f"refinement: {n / wall:.0f} patterns/s, median error {out['disorientation_deg_quantiles_before'][0]} -> "
+        f"{out['disorientation_deg_quantiles_after'][0]} deg, step kernel {step_ms:.2f} ms")
+    return out
 
 
 def learned_form(a, mp, dico, dev):
