@@ -26,6 +26,7 @@ SOURCES = {
     "noise.hip": [],
     "nn.hip": [],
     "local.hip": [],
+    "nlpar.hip": [],
 }
 
 COMMON_FLAGS = [

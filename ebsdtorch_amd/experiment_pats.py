@@ -73,6 +73,16 @@ class ExperimentPatterns(torch.nn.Module):
             pats = torch.nn.functional.avg_pool2d(pats, binning)
         return pats
 
+    def do_nlpar(self, k_rad: int = 3, coeff: float = 0.375) -> None:
+        """NLPAR-denoise the scan in place (experiment_pats.py:296-316,
+        preprocessing/nlpar.py); needs a 2D scan on the GPU."""
+        from .preprocessing import nlpar
+
+        if len(self.spatial_shape) != 2:
+            raise ValueError(f"NLPAR only supported for 2D EBSD scans, got {self.spatial_shape}")
+        self.patterns = nlpar(self.patterns.view(tuple(self.spatial_shape) + tuple(self.pattern_shape)),
+                              k_rad=k_rad, coeff=coeff).view(-1, *self.pattern_shape)
+
     def get_indices_per_phase(self) -> List[torch.Tensor]:
         if self.phases is None:
             raise ValueError("Phases must be indexed before retrieving indices per phase.")

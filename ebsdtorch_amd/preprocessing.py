@@ -1,0 +1,107 @@
+"""Pattern preprocessing of a scan: NLPAR (preprocessing/nlpar.py).
+
+`nlpar` mirrors nlpar.py:20-192 (Brewick, Wright & Rowenhorst 2019): the
+squared distance of every pattern to each pattern of its (2k+1)^2 window
+(ebsd_nlpar_distances), a per-point noise estimate from the 8 nearest, the
+normalised distances turned into logits whose temperature is fitted by
+Newton's method so the mean softmax weight of the centre over the 3x3 window
+equals `coeff`, and the softmax-weighted average of the window
+(ebsd_nlpar_average). The two passes over the patterns are the HIP kernels;
+the weights in between, (points x window) values, are the reference's own
+expressions on the device.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from ._lib import check, require_device
+
+
+def nlpar_offsets(k_rad: int) -> torch.Tensor:
+    """(n, 2) int64 window offsets (dy, dx) in the reference's order:
+    meshgrid 'ij' sorted by L1 length with torch.argsort on the CPU
+    (nlpar.py:83-98); index 0 is (0, 0) and 1..8 the nearest."""
+    offs = torch.stack(torch.meshgrid(torch.arange(-k_rad, k_rad + 1), torch.arange(-k_rad, k_rad + 1),
+                                      indexing="ij"), dim=-1).reshape(-1, 2)
+    return offs[torch.argsort(torch.sum(torch.abs(offs), dim=-1))]
+
+
+def _scan(data: torch.Tensor, k_rad: int) -> torch.Tensor:
+    require_device(data, "data")
+    if data.dim() != 4:
+        raise ValueError("The EBSD pattern data tensor must have 4 dimensions.")
+    if not 1 <= k_rad <= 5:
+        raise ValueError("k_rad must be in [1, 5]")
+    return data.to(torch.float32).contiguous()
+
+
+def nlpar_distances(data: torch.Tensor, k_rad: int) -> torch.Tensor:
+    """(H_s, W_s, n_off) float32: squared distance of each pattern of the
+    scan data (H_s, W_s, h, w) to the patterns at the window offsets
+    (nlpar_offsets order; zero patterns outside the scan), nlpar.py:105-117."""
+    x = _scan(data, k_rad)
+    h_scan, w_scan, h_pat, w_pat = x.shape
+    offs = nlpar_offsets(k_rad).to(x.device, torch.int32).contiguous()
+    dists = torch.empty((h_scan, w_scan, offs.shape[0]), dtype=torch.float32, device=x.device)
+    check(_lib.load().ebsd_nlpar_distances(x.data_ptr(), h_scan, w_scan, h_pat * w_pat, k_rad, offs.data_ptr(),
+                                           dists.data_ptr(), _lib.stream_ptr(x.device)), "ebsd_nlpar_distances")
+    return dists
+
+
+def nlpar(data: torch.Tensor, k_rad: int = 5, coeff: float = 0.375, coeff_fit_tol: float = 1e-6,
+          center_logit_bias: float = 0.0) -> torch.Tensor:
+    """NLPAR denoising of a scan data (H_s, W_s, h, w) float32 on the GPU
+    (nlpar.py:20-192); k_rad in [1, 5]. Returns the denoised scan."""
+    x = _scan(data, k_rad)
+    h_scan, w_scan, h_pat, w_pat = x.shape
+    n_pix = h_pat * w_pat
+    dev = x.device
+    offs = nlpar_offsets(k_rad)
+    n_off = offs.shape[0]
+    offs_dev = offs.to(dev, torch.int32).contiguous()
+    dists = nlpar_distances(x, k_rad)
+
+    # the reference's padded-point mask (nlpar.py:50-59, :109-117)
+    iy = torch.arange(h_scan, device=dev)[:, None, None] + offs_dev[:, 0].long()
+    ix = torch.arange(w_scan, device=dev)[None, :, None] + offs_dev[:, 1].long()
+    out_of_bounds = (iy < 0) | (iy >= h_scan) | (ix < 0) | (ix >= w_scan)
+
+    # noise level per point and normalised distances (nlpar.py:119-138)
+    sigma_hat = torch.sqrt(torch.min(dists[:, :, 1:9], dim=-1).values / (2 * n_pix))
+    sigma_hat_padded = torch.nn.functional.pad(sigma_hat, (k_rad,) * 4, mode="constant", value=1.0)
+    for i in range(n_off):
+        dy, dx = int(offs[i, 0]), int(offs[i, 1])
+        sigma_other = sigma_hat_padded[k_rad + dy:k_rad + dy + h_scan, k_rad + dx:k_rad + dx + w_scan]
+        dists[:, :, i] = (dists[:, :, i] - n_pix * (sigma_hat ** 2 + sigma_other ** 2)) / (
+            2 ** 0.5 * n_pix ** 0.5 * (sigma_hat ** 2 + sigma_other ** 2))
+    weights = -dists.clamp_min_(0.0)
+
+    # temperature fitted by Newton's method (nlpar.py:143-171)
+    # (the temperature becomes a float32 device scalar after the first step,
+    # as in the reference)
+    guess = 1.0
+    nn_weights = weights[:, :, :9]
+    for _ in range(10):
+        gsq = guess ** 2
+        obj = torch.mean(torch.softmax(nn_weights / gsq, dim=-1)[:, :, 0]) - coeff
+        e = torch.exp(nn_weights / gsq)
+        grad = torch.mean(
+            -2 * torch.exp(nn_weights[:, :, 0] / gsq)
+            * (torch.sum(nn_weights[:, :, [0]] * e, dim=-1) - torch.sum(nn_weights * e, dim=-1))
+            / (guess ** 3 * torch.sum(e, dim=-1) ** 2))
+        guess = guess - obj / grad
+        if bool(torch.abs(obj) < coeff_fit_tol):
+            break
+    weights /= guess ** 2
+    if center_logit_bias > 0:
+        weights[:, :, 0] += center_logit_bias
+    weights = weights.masked_fill_(out_of_bounds, -torch.inf)
+    weights = torch.nn.functional.softmax(weights, dim=-1).contiguous()
+
+    out = torch.empty_like(x)
+    check(_lib.load().ebsd_nlpar_average(x.data_ptr(), h_scan, w_scan, n_pix, k_rad, offs_dev.data_ptr(),
+                                         weights.data_ptr(), out.data_ptr(), _lib.stream_ptr(dev)),
+          "ebsd_nlpar_average")
+    return out

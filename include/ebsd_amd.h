@@ -384,6 +384,26 @@ int ebsd_grid_refine_step(const void* quads, int quad_dtype, int mp_rows, int mp
                           int n_levels, const int* level, const float* ops, int n_ops,
                           float* best_val, int* best_idx, float* best_q, void* stream);
 
+/*
+ * NLPAR (non-local pattern averaging) of a 2D scan x (scan_h, scan_w, n_pix)
+ * f32, window radius k_rad in [1, 5], offsets (n_off, 2) int32 (dy, dx) with
+ * |dy|, |dx| <= k_rad, n_off = (2 k_rad + 1)^2, in the caller's order
+ * (clamped to the window). Points outside the scan are zero patterns.
+ *   ebsd_nlpar_distances  dists (scan_h * scan_w, n_off) f32:
+ *                         sum over pixels of (x[p + offset] - x[p])^2
+ *   ebsd_nlpar_average    out (scan_h, scan_w, n_pix) f32:
+ *                         sum over offsets, in order, of x[p + offset] *
+ *                         weights[p][offset] (weights (points, n_off) f32)
+ * Replaces: the two loops over the window of preprocessing/nlpar.py:105-117
+ * (squared differences) and :187-191 (weighted average); the weights between
+ * them (nlpar.py:119-184) are computed by the caller.
+ */
+int ebsd_nlpar_distances(const float* x, int scan_h, int scan_w, int n_pix, int k_rad,
+                         const int* offsets, float* dists, void* stream);
+int ebsd_nlpar_average(const float* x, int scan_h, int scan_w, int n_pix, int k_rad,
+                       const int* offsets, const float* weights, float* out,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif
