@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--agree-sample", type=int, default=16384,
                    help="patterns re-indexed in plain FP32 to measure top-1 agreement")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
+    p.add_argument("--nlpar", type=int, default=1, help="measure NLPAR denoising of the batch as a scan (0 = skip)")
     p.add_argument("--refine", type=int, default=1, help="measure grid refinement of the indexed batch (0 = skip)")
     p.add_argument("--learned-train", type=int, default=2_000_000,
                    help="training patterns of the learned-proposal form (0 = skip that form)")
@@ -186,6 +187,10 @@ def main():
     if a.refine:
         refinement = refinement_form(a, mp, geom, exp, ori, eng.orientations[idx[:, 0].long()], det, dev)
 
+    nlpar_res = None
+    if a.nlpar:
+        nlpar_res = nlpar_form(a, exp, det, dev)
+
     learned = None
     if a.learned_train > 0:
         del eng
@@ -250,10 +255,67 @@ def main():
             "frac": round(proj_gbps / PEAKS["hbm_GBps"], 4), "bytes": proj_bytes,
         },
         "refinement": refinement,
+        "nlpar": nlpar_res,
         "learned_proposal": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def nlpar_form(a, exp, det, dev):
+    """NLPAR denoising (preprocessing/nlpar.py, k_rad 3 as
+    ExperimentPatterns.do_nlpar) of the batch viewed as a 256 x 256 scan of
+    60 x 60 patterns: the whole call, the two pattern passes with their
+    algorithmic HBM bytes (each reads the scan once and writes the
+    distances / the denoised scan), and the oracle on a 128 x 128 CPU sample."""
+    import torch
+
+    from ebsdtorch_amd.preprocessing import nlpar, nlpar_distances
+
+    n = exp.shape[0]
+    side = int(round(math.sqrt(n)))
+    if side * side != n:
+        return None
+    scan = exp.view(side, side, det, det)
+    k = 3
+    nlpar(scan[:32, :32].contiguous(), k_rad=k)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = nlpar(scan, k_rad=k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    e[0].record()
+    nlpar_distances(scan, k)
+    e[1].record()
+    torch.cuda.synchronize()
+    dist_ms = e[0].elapsed_time(e[1])
+    n_off = (2 * k + 1) ** 2
+    data_bytes = n * det * det * 4
+    dist_bytes = data_bytes + n * n_off * 4
+    cpu = None
+    if a.cpu_seconds > 0:
+        from oracle import nlpar_oracle as N
+
+        sample = scan[:128, :128].cpu()
+        t1 = time.perf_counter()
+        N.nlpar(sample, k_rad=k)
+        tc = time.perf_counter() - t1
+        cpu = {"patterns_per_s": round(128 * 128 / tc, 1), "cores": torch.get_num_threads(), "kind": "port",
+               "sample": f"128 x 128 scan ({tc:.2f} s)"}
+    del out
+    res = {
+        "workload": f"NLPAR k_rad {k} ({n_off} neighbours) of a {side} x {side} scan of {det}x{det} patterns",
+        "patterns_per_s": round(n / wall, 1),
+        "ms": round(wall * 1000, 2),
+        "distance_pass": {"ms": round(dist_ms, 3), "algorithmic_bytes": dist_bytes,
+                          "achieved_GBps": round(dist_bytes / (dist_ms / 1000) / 1e9, 1),
+                          "hbm_peak_GBps": PEAKS["hbm_GBps"]},
+        "cpu_oracle": cpu,
+    }
+    log(f"nlpar: {n / wall:.0f} patterns/s ({wall * 1000:.1f} ms), distance pass {dist_ms:.2f} ms "
+        f"({res['distance_pass']['achieved_GBps']} GB/s)")
+    return res
 
 
 def refinement_form(a, mp, geom, exp, truth, found, det, dev):
