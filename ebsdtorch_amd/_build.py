@@ -27,6 +27,7 @@ SOURCES = {
     "nn.hip": [],
     "local.hip": [],
     "nlpar.hip": [],
+    "preprocess.hip": [],
 }
 
 COMMON_FLAGS = [

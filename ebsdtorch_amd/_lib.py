@@ -115,6 +115,9 @@ _SIGNATURES = {
     "ebsd_nlpar_distances": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p]),
     "ebsd_nlpar_average": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,
                                     _c_void_p]),
+    "ebsd_clahe": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_float, _c_void_p,
+                            _c_void_p]),
+    "ebsd_normalize_patterns": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p]),
     "ebsd_nn_col2im": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p]),
     "ebsd_nn_bias_act": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_void_p, _c_int, _c_void_p]),
     "ebsd_nn_relu_bwd": (_c_int, [_c_void_p, _c_void_p, ctypes.c_longlong, _c_void_p]),

@@ -73,6 +73,32 @@ class ExperimentPatterns(torch.nn.Module):
             pats = torch.nn.functional.avg_pool2d(pats, binning)
         return pats
 
+    def subtract_overall_background(self) -> None:
+        """Subtract the mean pattern of the scan (experiment_pats.py:223-232)."""
+        self.patterns -= torch.mean(self.patterns, dim=0, keepdim=True)
+
+    def contrast_enhance_clahe(self, clip_limit: float = 40.0, tile_grid_size: int = 4, n_bins: int = 64) -> None:
+        """CLAHE of every pattern (experiment_pats.py:234-254, clahe.py)."""
+        from .preprocessing import clahe_grayscale
+
+        self.patterns = clahe_grayscale(self.patterns, clip_limit=clip_limit, n_bins=n_bins,
+                                        grid_shape=(tile_grid_size, tile_grid_size))
+
+    def normalize_per_pattern(self, norm_type: str = "minmax") -> None:
+        """Per-pattern normalisation (experiment_pats.py:256-281)."""
+        from .preprocessing import normalize_patterns
+
+        self.patterns = self.patterns.to(torch.float32).contiguous()
+        normalize_patterns(self.patterns.view(self.n_patterns, -1), norm_type)
+
+    def standard_clean(self) -> None:
+        """Background subtraction, minmax, CLAHE, minmax
+        (experiment_pats.py:283-293)."""
+        self.subtract_overall_background()
+        self.normalize_per_pattern(norm_type="minmax")
+        self.contrast_enhance_clahe()
+        self.normalize_per_pattern(norm_type="minmax")
+
     def do_nlpar(self, k_rad: int = 3, coeff: float = 0.375) -> None:
         """NLPAR-denoise the scan in place (experiment_pats.py:296-316,
         preprocessing/nlpar.py); needs a 2D scan on the GPU."""

@@ -1,4 +1,6 @@
-"""Pattern preprocessing of a scan: NLPAR (preprocessing/nlpar.py).
+"""Pattern preprocessing: NLPAR (preprocessing/nlpar.py), CLAHE
+(preprocessing/clahe.py) and per-pattern normalisation
+(ebsd/experiment_pats.py:256-281).
 
 `nlpar` mirrors nlpar.py:20-192 (Brewick, Wright & Rowenhorst 2019): the
 squared distance of every pattern to each pattern of its (2k+1)^2 window
@@ -104,4 +106,36 @@ def nlpar(data: torch.Tensor, k_rad: int = 5, coeff: float = 0.375, coeff_fit_to
     check(_lib.load().ebsd_nlpar_average(x.data_ptr(), h_scan, w_scan, n_pix, k_rad, offs_dev.data_ptr(),
                                          weights.data_ptr(), out.data_ptr(), _lib.stream_ptr(dev)),
           "ebsd_nlpar_average")
+    return out
+
+
+NORM_MODES = {"minmax": 0, "zeromean": 1, "standard": 2}
+
+
+def normalize_patterns(x: torch.Tensor, norm_type: str = "minmax") -> torch.Tensor:
+    """Normalise each pattern (row of x viewed as (N, -1), float32 on the GPU)
+    in place: "minmax" (x - min) / (1e-4 + max - min), "zeromean", or
+    "standard" (zero mean, unit unbiased std) (experiment_pats.py:256-281)."""
+    require_device(x, "x")
+    if norm_type not in NORM_MODES:
+        raise ValueError(f"Invalid normalization method. Got {norm_type} but expected 'minmax', "
+                         "'zeromean', or 'standard'.")
+    if x.dtype != torch.float32 or not x.is_contiguous():
+        raise ValueError("patterns must be contiguous float32")
+    n = x.shape[0]
+    check(_lib.load().ebsd_normalize_patterns(x.data_ptr(), n, x.numel() // max(n, 1), NORM_MODES[norm_type],
+                                              _lib.stream_ptr(x.device)), "ebsd_normalize_patterns")
+    return x
+
+
+def clahe_grayscale(x: torch.Tensor, clip_limit: float = 40.0, n_bins: int = 64, grid_shape=(4, 4)) -> torch.Tensor:
+    """CLAHE of images x (..., H, W) float32 on the GPU, values in [0, 1]
+    (clahe.py:18-130). Returns a new tensor of the same shape."""
+    require_device(x, "x")
+    h, w = x.shape[-2:]
+    src = x.to(torch.float32).contiguous()
+    n = src.numel() // (h * w)
+    out = torch.empty_like(src)
+    check(_lib.load().ebsd_clahe(src.data_ptr(), n, h, w, int(grid_shape[0]), int(grid_shape[1]), int(n_bins),
+                                 float(clip_limit), out.data_ptr(), _lib.stream_ptr(src.device)), "ebsd_clahe")
     return out

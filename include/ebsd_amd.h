@@ -404,6 +404,28 @@ int ebsd_nlpar_average(const float* x, int scan_h, int scan_w, int n_pix, int k_
                        const int* offsets, const float* weights, float* out,
                        void* stream);
 
+/*
+ * CLAHE of n patterns x (n, h, w) f32 (values in [0, 1]; others are counted
+ * in the end bins) on a grid_h x grid_w tile grid (grid_h * grid_w <= 64)
+ * with n_bins <= 64 and clip_limit (<= 0: no clipping): reflect padding to a
+ * multiple of the grid, per-tile histograms of trunc(v (n_bins - 1)),
+ * clipping at max(clip_limit * tile_pixels // n_bins, 1) with the excess
+ * redistributed, clamped CDFs, and each output pixel the trilinear
+ * (row, column, value) lookup of the CDF volume (align_corners=False,
+ * border). out (n, h, w) f32; the padding must be smaller than h and w.
+ * Replaces: preprocessing/clahe.py:18-130 (clahe_grayscale).
+ */
+int ebsd_clahe(const float* x, int n, int h, int w, int grid_h, int grid_w, int n_bins,
+               float clip_limit, float* out, void* stream);
+
+/*
+ * Per-pattern normalisation in place of x (n, n_pix) f32: mode 0 minmax
+ * (x - min) / (1e-4 + max - min), 1 zero mean, 2 standard (zero mean, unit
+ * unbiased standard deviation).
+ * Replaces: ebsd/experiment_pats.py:256-281 (normalize_per_pattern).
+ */
+int ebsd_normalize_patterns(float* x, int n, int n_pix, int mode, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
