@@ -71,13 +71,13 @@ def nlpar(data: torch.Tensor, k_rad: int = 5, coeff: float = 0.375, coeff_fit_to
     out_of_bounds = (iy < 0) | (iy >= h_scan) | (ix < 0) | (ix >= w_scan)
 
     # noise level per point and normalised distances (nlpar.py:119-138)
+    # (every offset at once: the same per-element expression as the
+    # reference's loop over the window)
     sigma_hat = torch.sqrt(torch.min(dists[:, :, 1:9], dim=-1).values / (2 * n_pix))
     sigma_hat_padded = torch.nn.functional.pad(sigma_hat, (k_rad,) * 4, mode="constant", value=1.0)
-    for i in range(n_off):
-        dy, dx = int(offs[i, 0]), int(offs[i, 1])
-        sigma_other = sigma_hat_padded[k_rad + dy:k_rad + dy + h_scan, k_rad + dx:k_rad + dx + w_scan]
-        dists[:, :, i] = (dists[:, :, i] - n_pix * (sigma_hat ** 2 + sigma_other ** 2)) / (
-            2 ** 0.5 * n_pix ** 0.5 * (sigma_hat ** 2 + sigma_other ** 2))
+    sigma_other = sigma_hat_padded[iy + k_rad, ix + k_rad]
+    s2 = sigma_hat[:, :, None] ** 2 + sigma_other ** 2
+    dists = (dists - n_pix * s2) / (2 ** 0.5 * n_pix ** 0.5 * s2)
     weights = -dists.clamp_min_(0.0)
 
     # temperature fitted by Newton's method (nlpar.py:143-171)
