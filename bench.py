@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--agree-sample", type=int, default=16384,
                    help="patterns re-indexed in plain FP32 to measure top-1 agreement")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
+    p.add_argument("--pc-fit", type=int, default=1, help="measure the pattern-centre fitter (0 = skip)")
     p.add_argument("--nlpar", type=int, default=1, help="measure NLPAR denoising of the batch as a scan (0 = skip)")
     p.add_argument("--refine", type=int, default=1, help="measure grid refinement of the indexed batch (0 = skip)")
     p.add_argument("--learned-train", type=int, default=2_000_000,
@@ -191,6 +192,10 @@ def main():
     if a.nlpar:
         nlpar_res = nlpar_form(a, exp, det, dev)
 
+    pcfit = None
+    if a.pc_fit:
+        pcfit = pc_fit_form(a, mp, exp, det, dev)
+
     learned = None
     if a.learned_train > 0:
         del eng
@@ -256,10 +261,62 @@ def main():
         },
         "refinement": refinement,
         "nlpar": nlpar_res,
+        "pc_fit": pcfit,
         "learned_proposal": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def pc_fit_form(a, mp, exp, det, dev):
+    """Spatial-FFT pattern-centre fit (fitting.py:359-535, metric "dot") of 8
+    patterns of the batch (simulated at centre (0.5, 0.5, 0.6)): pc_z from
+    0.5 to 0.7 in 8 steps, centre search +-0.1 (a 72 x 72 extended detector),
+    a 3-degree m-3m dictionary; wall time of the call, and the oracle on the
+    same inputs on the host cores."""
+    import torch
+
+    from ebsdtorch_amd.fitting import spatial_fft_pc_fitter
+    from ebsdtorch_amd.master_pattern import MasterPattern
+    from ebsdtorch_amd.orientations import sample_ori_fz_laue_angle
+
+    pats = exp[:8].view(8, 1, det, det).clone()
+    quats = sample_ori_fz_laue_angle(11, 3.0, dev, permute=False)
+    kw = dict(pc_x_min=0.4, pc_x_max=0.6, pc_y_min=0.4, pc_y_max=0.6, pc_z_min=0.5, pc_z_max=0.7, pc_z_steps=8)
+    mpo = MasterPattern(mp, laue_group=11)
+    spatial_fft_pc_fitter(mpo, pats.clone(), quats_dict=quats, dictionary_indexing_batch_size=1024, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pc, _ = spatial_fft_pc_fitter(mpo, pats.clone(), quats_dict=quats, dictionary_indexing_batch_size=1024, **kw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cpu = None
+    if a.cpu_seconds > 0:
+        from oracle import fitting_oracle as FO
+
+        t1 = time.perf_counter()
+        ref_pc, _ = FO.spatial_fft_pc_fitter(mp.cpu(), quats.cpu(), pats.cpu(), 0.4, 0.6, 0.4, 0.6, 0.5, 0.7, 8,
+                                             batch_size=1024)
+        tc = time.perf_counter() - t1
+        cpu = {"seconds": round(tc, 2), "cores": torch.get_num_threads(), "kind": "port",
+               "same_centre": bool(torch.allclose(ref_pc, pc.cpu(), atol=1e-6))}
+    # the reference's defaults: 1-degree dictionary, 16 pc_z steps from 0.3 to 0.7
+    q1 = sample_ori_fz_laue_angle(11, 1.0, dev, permute=False)
+    kw1 = dict(kw, pc_z_min=0.3, pc_z_max=0.7, pc_z_steps=16)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    pc1, _ = spatial_fft_pc_fitter(mpo, pats.clone(), quats_dict=q1, dictionary_indexing_batch_size=1024, **kw1)
+    torch.cuda.synchronize()
+    wall1 = time.perf_counter() - t2
+    res = {"workload": f"pattern-centre fit of 8 {det}x{det} patterns, {quats.shape[0]} orientations (3 deg), "
+                       f"8 pc_z steps, centre range +-0.1",
+           "seconds": round(wall, 3), "centre": [round(v, 4) for v in pc.tolist()], "cpu_oracle": cpu,
+           "reference_defaults": {"workload": f"{q1.shape[0]} orientations (1 deg), 16 pc_z steps 0.3-0.7",
+                                  "seconds": round(wall1, 3), "centre": [round(v, 4) for v in pc1.tolist()],
+                                  "dictionary_patterns_per_s": round(q1.shape[0] * 16 / wall1, 1)}}
+    log(f"pc fit: {wall:.2f} s, centre {res['centre']}" + (f", CPU oracle {cpu['seconds']} s" if cpu else "")
+        + f"; reference defaults ({q1.shape[0]} orientations x 16 pc_z) {wall1:.2f} s, centre {res['reference_defaults']['centre']}")
+    return res
 
 
 def nlpar_form(a, exp, det, dev):
