@@ -124,11 +124,9 @@ __global__ __launch_bounds__(256) void im2col8_kernel(const float* __restrict__ 
 // bias + ReLU: Y[row][co] = max(0, bias[co] + sum_t w[co][t] in_t) over the
 // 9 taps t = ky*3 + kx, in_t = round_T(scale * X[2 oy + ky - 1][2 ox + kx - 1])
 // (0 outside the image): the values of im2col<T> + the GEMM + bias_act, up to
-// the order of the fp32 sum. One thread per (row, 4 output channels), so a
-// wave stores consecutive 16-byte pieces of the NHWC rows; the taps are read
-// by the c_out / 4 threads of a row from cache. Weights and biases are staged
-// in LDS as 12 floats per channel (9 taps, bias, 2 pad), once per workgroup
-// of a grid-stride launch (8 workgroups per CU).
+// the order of the fp32 sum. One thread per output pixel (row): its taps and
+// index arithmetic once, then all c_out channels with the weights and biases
+// as wave-uniform (scalar) loads; grid-stride launch of 8 workgroups per CU.
 template <typename T>
 __device__ __forceinline__ float round_to(float v);
 template <>
@@ -145,51 +143,61 @@ __device__ __forceinline__ float load_float<__hip_bfloat16>(const __hip_bfloat16
 
 constexpr int kConvC1MaxOut = 256;
 
+// The 9 weights of a channel row as floats. BF16 rows are read as 5 dwords
+// (two weights each; the row is 4-byte aligned and its 10th entry is padding)
+// so that wave-uniform rows become scalar loads.
+template <typename T>
+__device__ __forceinline__ void load_taps(const T* __restrict__ w, float (&out)[9]);
+template <>
+__device__ __forceinline__ void load_taps<float>(const float* __restrict__ w, float (&out)[9]) {
+#pragma unroll
+  for (int t = 0; t < 9; ++t) out[t] = w[t];
+}
+template <>
+__device__ __forceinline__ void load_taps<__hip_bfloat16>(const __hip_bfloat16* __restrict__ w, float (&out)[9]) {
+  const uint32_t* d = reinterpret_cast<const uint32_t*>(w);
+#pragma unroll
+  for (int t = 0; t < 5; ++t) {
+    const uint32_t v = d[t];
+    out[2 * t] = __uint_as_float(v << 16);
+    if (2 * t + 1 < 9) out[2 * t + 1] = __uint_as_float(v & 0xffff0000u);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void conv_c1_kernel(const float* __restrict__ X, int H, int W, int Ho, int Wo,
                                                       float scale, const T* __restrict__ wgt, int ld_w,
                                                       const float* __restrict__ bias, int c_out,
-                                                      float* __restrict__ Y, int n_units) {
-  __shared__ __attribute__((aligned(16))) float ws[kConvC1MaxOut * 12];
-  for (int e = threadIdx.x; e < c_out * 12; e += 256) {
-    const int co = e / 12, t = e - 12 * (e / 12);
-    ws[e] = t < 9 ? load_float<T>(wgt + (size_t)co * ld_w + t) : (t == 9 ? bias[co] : 0.0f);
-  }
-  __syncthreads();
-  const int q4 = c_out >> 2;
-  for (int t = blockIdx.x * 256 + threadIdx.x; t < n_units; t += gridDim.x * 256) {
-  const int row = t / q4;
-  const int co = (t - row * q4) << 2;
-  const int ox = row % Wo;
-  const int r2 = row / Wo;
-  const int oy = r2 % Ho;
-  const int b = r2 / Ho;
-  const float* __restrict__ img = X + (int64_t)b * H * W;
-  float in[9];
+                                                      float* __restrict__ Y, int n_rows) {
+  for (int row = blockIdx.x * 256 + threadIdx.x; row < n_rows; row += gridDim.x * 256) {
+    const int ox = row % Wo;
+    const int r2 = row / Wo;
+    const int oy = r2 % Ho;
+    const int b = r2 / Ho;
+    const float* __restrict__ img = X + (int64_t)b * H * W;
+    float in[9];
 #pragma unroll
-  for (int ky = 0; ky < 3; ++ky)
+    for (int ky = 0; ky < 3; ++ky)
 #pragma unroll
-    for (int kx = 0; kx < 3; ++kx) {
-      const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
-      in[ky * 3 + kx] = (y >= 0 && y < H && x >= 0 && x < W) ? round_to<T>(scale * img[y * W + x]) : 0.0f;
+      for (int kx = 0; kx < 3; ++kx) {
+        const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+        in[ky * 3 + kx] = (y >= 0 && y < H && x >= 0 && x < W) ? round_to<T>(scale * img[y * W + x]) : 0.0f;
+      }
+    float* __restrict__ out = Y + (int64_t)row * c_out;
+    // co is wave-uniform: the weights and biases are scalar loads
+    for (int co = 0; co < c_out; co += 4) {
+      float r[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float wv[9];
+        load_taps<T>(wgt + (size_t)(co + q) * ld_w, wv);
+        float s = wv[0] * in[0];
+#pragma unroll
+        for (int t = 1; t < 9; ++t) s = fmaf(wv[t], in[t], s);
+        r[q] = fmaxf(s + bias[co + q], 0.0f);
+      }
+      *reinterpret_cast<float4*>(out + co) = make_float4(r[0], r[1], r[2], r[3]);
     }
-  float r[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float4* wp = reinterpret_cast<const float4*>(ws + (co + q) * 12);
-    const float4 a = wp[0], c = wp[1], d = wp[2];
-    float s = a.x * in[0];
-    s = fmaf(a.y, in[1], s);
-    s = fmaf(a.z, in[2], s);
-    s = fmaf(a.w, in[3], s);
-    s = fmaf(c.x, in[4], s);
-    s = fmaf(c.y, in[5], s);
-    s = fmaf(c.z, in[6], s);
-    s = fmaf(c.w, in[7], s);
-    s = fmaf(d.x, in[8], s);
-    r[q] = fmaxf(s + d.y, 0.0f);
-  }
-  *reinterpret_cast<float4*>(Y + (int64_t)row * c_out + co) = make_float4(r[0], r[1], r[2], r[3]);
   }
 }
 
@@ -643,8 +651,10 @@ extern "C" int ebsd_nn_conv_c1(const float* x, int b, int h, int w, float scale,
   if (b < 1 || h < 1 || w < 1 || ld_w < 9 || c_out < 4 || c_out > kConvC1MaxOut || (c_out & 3) != 0)
     return EBSD_ERR_SHAPE;
   if ((reinterpret_cast<uintptr_t>(y) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (dtype == EBSD_DTYPE_BF16 && (ld_w < 10 || (ld_w & 1) != 0 || (reinterpret_cast<uintptr_t>(weight) & 3) != 0))
+    return EBSD_ERR_ALIGN;  // BF16 rows are read as dwords, the 10th entry included
   const int ho = (h + 1) / 2, wo = (w + 1) / 2;
-  const int64_t units = (int64_t)b * ho * wo * (c_out / 4);
+  const int64_t units = (int64_t)b * ho * wo;
   if (units >= INT_MAX - (1 << 20)) return EBSD_ERR_SHAPE;
   const dim3 g((unsigned)std::min<int64_t>((units + 255) / 256, 2048));
   hipStream_t st = static_cast<hipStream_t>(stream);
