@@ -44,6 +44,11 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--precision", default="fp8", choices=["fp32", "bf16", "fp16", "fp8"])
     p.add_argument("--rescore-k", type=int, default=16)
+    p.add_argument("--search-components", default="auto",
+                   help="low-precision forms: search the dictionary's coordinates in its leading principal "
+                        "directions ('auto': enough for --explained-variance; an int; 0: full patterns), "
+                        "then rescore the candidates exactly in FP32")
+    p.add_argument("--explained-variance", type=float, default=0.995)
     p.add_argument("--dict-size", type=int, default=1_000_000)
     p.add_argument("--batch", type=int, default=65536)
     p.add_argument("--topk", type=int, default=16)
@@ -106,24 +111,35 @@ def main():
 
     # ---- dictionary projection (timed separately: once per dictionary) ----
     rk = a.rescore_k if a.precision != "fp32" else 0
+    sc = 0 if (a.precision == "fp32" or rk == 0 or a.search_components == "0") else (
+        "auto" if a.search_components == "auto" else int(a.search_components))
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     eng = None
     proj_ms = []
     for rep in range(2):
+        eng = None
+        torch.cuda.empty_cache()
         ev[0].record()
-        eng = E.DictionaryIndexer(mp, rays, dico, precision=a.precision, rescore_k=rk)
+        eng = E.DictionaryIndexer(mp, rays, dico, precision=a.precision, rescore_k=rk, search_components=sc,
+                                  explained_variance=a.explained_variance)
         ev[1].record()
         torch.cuda.synchronize()
         proj_ms.append(ev[0].elapsed_time(ev[1]))
     n_dict = eng.size
-    # time the projection kernel alone on the search dtype
+    search_dim = eng.search_components if eng.search_components else D
+    # time the projection kernel alone on the dtype it writes (with a PCA
+    # search: the FP32 dictionary the basis and the rescoring come from)
+    proj_dtype = DTYPE_F32 if eng.search_components else eng.dtype
+    proj_out = eng.dictionary_f32 if eng.search_components else eng.dictionary
     pe = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     pe[0].record()
-    kernels.project(mp, rays, dico, eng.dtype, out=eng.dictionary, quads=eng.quads)
+    kernels.project(mp, rays, dico, proj_dtype, out=proj_out, quads=eng.quads)
     pe[1].record()
     torch.cuda.synchronize()
     proj_kernel_ms = pe[0].elapsed_time(pe[1])
-    log(f"dictionary projection: {proj_ms[-1]:.1f} ms (engine), kernel {proj_kernel_ms:.2f} ms")
+    log(f"dictionary build: {proj_ms[-1]:.1f} ms (engine), projection kernel {proj_kernel_ms:.2f} ms"
+        + (f"; search on {eng.search_components} principal components ({eng.explained:.4f} of the second moment)"
+           if eng.search_components else ""))
 
     need_f32 = eng.dictionary_f32 is not None
 
@@ -132,8 +148,7 @@ def main():
     sim_ms = []
 
     def timed_step():
-        q = eng.prepare(exp)
-        qf = eng.prepare(exp, dtype=DTYPE_F32) if need_f32 else None
+        q, qf = eng.prepare_pair(exp)
         k_search = max(a.topk, rk) if need_f32 else a.topk
         val = torch.empty((q.rows, k_search), dtype=torch.float32, device=dev)
         idx = torch.empty((q.rows, k_search), dtype=torch.int32, device=dev)
@@ -177,10 +192,10 @@ def main():
         del ref
         log(f"top-1 agreement with FP32 over {ns} patterns: {agree:.5f}; FP32 DI {fp32_rate:.0f} patterns/s")
 
-    flops = 2.0 * a.batch * n_dict * D
+    flops = 2.0 * a.batch * n_dict * search_dim
     achieved = flops / (sim_avg_ms / 1000) / 1e12
     peak = PEAKS[a.precision]
-    out_bytes = {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[a.precision]
+    out_bytes = 4 if eng.search_components else {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[a.precision]
     proj_bytes = n_dict * (D * out_bytes + 16) + 2 * a.mp_size * a.mp_size * 4 + 12 * D
     proj_gbps = proj_bytes / (proj_kernel_ms / 1000) / 1e9
 
@@ -209,10 +224,12 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "sim_topk_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            t = json.load(f)
-        if (t.get("precision") == a.precision and t.get("dictionary") == n_dict and t.get("batch") == a.batch
-                and t.get("detector") == [det, det] and t.get("topk") == a.topk):
-            traffic = t["traffic_bytes"]
+            recs = json.load(f)
+        for t in (recs if isinstance(recs, list) else [recs]):
+            if (t.get("precision") == a.precision and t.get("dictionary") == n_dict and t.get("batch") == a.batch
+                    and t.get("detector") == [det, det] and t.get("topk") == a.topk
+                    and t.get("search_dim", D) == search_dim):
+                traffic = t["traffic_bytes"]
 
     cpu = None
     if a.cpu_seconds > 0:
@@ -236,6 +253,9 @@ def main():
             "workload": f"DI {n_dict} m-3m dictionary x {a.batch}-pattern batch, {det}x{det} detector, top-{a.topk}",
             "dictionary": n_dict, "batch": a.batch, "detector": [det, det], "topk": a.topk,
             "precision": a.precision, "rescore_k": rk,
+            "search": ({"pca_components": eng.search_components, "second_moment_fraction": round(eng.explained, 5),
+                        "rescore": f"top-{max(rk, a.topk)} exactly in FP32 on the full {D} pixels"}
+                       if eng.search_components else f"full {D} pixels"),
         },
         "top1_agreement_vs_fp32": 1.0 if a.precision == "fp32" else agree,
         "agreement_sample": None if a.precision == "fp32" else min(a.agree_sample, a.batch),

@@ -73,9 +73,12 @@ class DictionaryIndexer(_ResidentSearch):
 
     def __init__(self, master_pattern: torch.Tensor, rays: torch.Tensor, orientations: torch.Tensor,
                  precision: str = "fp32", rescore_k: int = 0, dict_unit_norm: bool = False,
-                 renorm_dirs: bool = False):
+                 renorm_dirs: bool = False, search_components: Union[int, str] = 0,
+                 explained_variance: float = 0.995, basis_sample: int = 1 << 18):
         if precision not in kernels.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(kernels.PRECISIONS)}")
+        if search_components and rescore_k <= 0:
+            raise ValueError("a PCA-compressed search needs rescore_k > 0 (exact FP32 rescoring)")
         self.device = orientations.device
         self.mp = master_pattern.to(self.device, torch.float32).contiguous()
         self.rays = rays.to(self.device, torch.float32).contiguous()
@@ -89,10 +92,78 @@ class DictionaryIndexer(_ResidentSearch):
         # the master pattern's quad table: one 16-byte gather per bilinear
         # sample in the projection (same values bit for bit)
         self.quads = kernels.mp_quad_table(self.mp)
+        self.basis_t = None
+        self.search_components = 0
+        self.explained = 1.0
+        if search_components:
+            self.dictionary_f32 = self._project(DTYPE_F32)
+            self._build_pca_search(search_components, float(explained_variance), int(basis_sample))
+            return
         self.dictionary = self._project(self.dtype)
         self.dictionary_f32 = None
         if self.rescore_k > 0 and self.dtype != DTYPE_F32:
             self.dictionary_f32 = self._project(DTYPE_F32)
+
+    def _build_pca_search(self, components, explained: float, sample: int, chunk: int = 1 << 16):
+        """PCA-compressed search: the candidates come from the dictionary's
+        coordinates in its k leading principal directions (uncentred second
+        moment of the zero-mean patterns, so the dot products themselves are
+        approximated), in the search dtype; rescore_k > 0 re-scores them
+        exactly against the full FP32 dictionary. The second moment is
+        accumulated on the matrix cores (ebsd_cov_update: centred aggregate,
+        plus n mean mean^T) over up to `sample` dictionary rows and solved in
+        FP64 on the host. components "auto": the smallest multiple of 64 (at least the
+        search kernel's 4 K stages) whose leading eigenvalues hold
+        `explained` of the trace; an int: that many."""
+        from .pca_covmat import OnlineCovMatrix
+
+        full = self.dictionary_f32
+        d = full.n_pix
+        cov = OnlineCovMatrix(d, device=self.device)
+        n_s = min(sample, full.rows)
+        for s in range(0, n_s, chunk):
+            cov(full.narrow(s, min(chunk, n_s - s)))
+        n_obs = float(int(cov.obs.item()))
+        mean = cov.mean.to(torch.float64)
+        m2 = cov._full_aggregate().to(torch.float64) + n_obs * torch.outer(mean, mean)
+        # one FP64 solve per dictionary, on the host (LAPACK): ~1-2 s for
+        # D = 3600, and no device eigensolver in the indexing process
+        evals, evecs = torch.linalg.eigh(m2.cpu())
+        evals, evecs = evals.clamp_min(0.0).to(self.device), evecs.to(self.device)
+        elem = kernels.TORCH_DTYPE[self.dtype].itemsize
+        k_min = max(64, -(-256 // elem))  # >= 4 K stages of 64 bytes
+        if components == "auto":
+            frac = torch.cumsum(evals.flip(0), 0) / evals.sum()
+            k = int(torch.searchsorted(frac, torch.tensor([explained], dtype=frac.dtype, device=frac.device))) + 1
+        else:
+            k = int(components)
+        k = min(max(-(-k // 64) * 64, k_min), d)
+        self.search_components = k
+        self.explained = float(evals[-k:].sum() / evals.sum())
+        basis = evecs[:, -k:].to(torch.float32)  # (d, k)
+        bt = kernels.alloc_patterns(k, d, DTYPE_F32, self.device)
+        bt.data.zero_()
+        bt.data[:, :d] = basis.T
+        self.basis_t = bt
+        coef = torch.empty((full.rows, kernels.padded_width(k, DTYPE_F32)), dtype=torch.float32, device=self.device)
+        for s in range(0, full.rows, chunk):
+            kernels.gemm_nt(full.narrow(s, min(chunk, full.rows - s)), bt, out=coef[s:s + chunk])
+        self.dictionary = kernels.prep_patterns(coef[:, :k], self.dtype, zero_mean=False)
+        del coef
+
+    def prepare_pair(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None, zero_mean: bool = True):
+        """(search rows, FP32 rows for rescoring or None) of (N, P) patterns:
+        with a PCA-compressed search the search rows are the patterns'
+        coordinates in the basis, in the search dtype."""
+        if self.basis_t is None:
+            q = self.prepare(patterns, pix_idx, zero_mean)
+            qf = self.prepare(patterns, pix_idx, zero_mean, DTYPE_F32) if self.dictionary_f32 is not None else None
+            return q, qf
+        qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, pix_idx=pix_idx)
+        k = self.search_components
+        qc = torch.empty((qf.rows, kernels.padded_width(k, DTYPE_F32)), dtype=torch.float32, device=self.device)
+        kernels.gemm_nt(qf, self.basis_t, out=qc)
+        return kernels.prep_patterns(qc[:, :k], self.dtype, zero_mean=False), qf
 
     def _project(self, dtype: int) -> kernels.PatternMatrix:
         return kernels.project(self.mp, self.rays, self.orientations, dtype, zero_mean=True, quads=self.quads,
@@ -104,7 +175,12 @@ class DictionaryIndexer(_ResidentSearch):
 
     def prepare(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None,
                 zero_mean: bool = True, dtype: Optional[int] = None) -> kernels.PatternMatrix:
-        """(N, P) float patterns -> zero-mean padded rows of the search dtype."""
+        """(N, P) float patterns -> zero-mean padded rows of the search dtype
+        (`dtype` given: full-width rows of that dtype). With a PCA-compressed
+        search the search rows are the basis coordinates (prepare_pair gives
+        both at once)."""
+        if dtype is None and self.basis_t is not None:
+            return self.prepare_pair(patterns, pix_idx, zero_mean)[0]
         return kernels.prep_patterns(patterns, self.dtype if dtype is None else dtype, zero_mean=zero_mean,
                                      pix_idx=pix_idx)
 

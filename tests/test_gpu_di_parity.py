@@ -134,3 +134,35 @@ def test_full_size_self_retrieval_333k():
         dis = E.disori_angle_laue(dico[pick[bad]].double(), dico[top1[bad]].double(), 11)
         assert torch.all(torch.rad2deg(dis) < 0.05), torch.rad2deg(dis)
     assert torch.all(val[:, 1:] <= val[:, :-1])
+
+
+@pytest.mark.parametrize("components", ["auto", 128])
+def test_pca_compressed_search_keeps_fp32_top1(components):
+    """FP8 search on the dictionary's leading principal coordinates, then an
+    exact FP32 rescore of the top 16 on the full patterns: the top-1 equals
+    plain FP32 DI's (up to score ties) on noisy patterns; the basis is
+    orthonormal and its second-moment fraction is as reported."""
+    mp = synthetic.band_master_pattern(201, 11, 40, 2.0, seed=0, device=DEV)
+    dico = E.sample_ori_fz_laue(11, 30_000, DEV, permute=True, generator=torch.Generator().manual_seed(1))
+    rays = E.EBSDGeometry((40, 40), proj_center=(0.5, 0.5, 0.6)).get_rays().to(DEV)
+    ori = synthetic.random_orientations(2048, seed=5, device=DEV)
+    exp = synthetic.simulate_patterns(mp, rays, ori, mean_counts=200.0, gaussian_sigma=0.05, seed=6)
+    _say("pca search", components)
+    ref = E.DictionaryIndexer(mp, rays, dico, precision="fp32")
+    _, ref_idx = ref.index(ref.prepare(exp), 1)
+    eng = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, search_components=components,
+                              explained_variance=0.995)
+    k = eng.search_components
+    assert k % 64 == 0 and 256 <= k <= 1600
+    b = eng.basis_t.data[:k, :1600].double()
+    torch.testing.assert_close(b @ b.T, torch.eye(k, dtype=torch.float64, device=DEV), atol=1e-5, rtol=0)
+    if components == "auto":
+        assert eng.explained >= 0.995
+    q, qf = eng.prepare_pair(exp)
+    val, idx = eng.index(q, 1, queries_f32=qf)
+    dict_pats = ref.dictionary.to_float().cpu()
+    n_diff = _assert_top1_parity(idx[:, 0].cpu().long(), ref_idx[:, 0].cpu().long(), exp.cpu(), dict_pats)
+    _say(f"pca search k={k} ({eng.explained:.4f}): {n_diff} top-1 differences (ties)")
+    # the scores are the exact FP32 ones of the reported entries
+    torch.testing.assert_close(val[:, 0], (qf.to_float() * ref.dictionary.to_float()[idx[:, 0].long()]).sum(-1),
+                               atol=1e-4, rtol=1e-5)
