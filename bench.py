@@ -127,6 +127,10 @@ def main():
         proj_ms.append(ev[0].elapsed_time(ev[1]))
     n_dict = eng.size
     search_dim = eng.search_components if eng.search_components else D
+    search_cfg = ({"pca_components": eng.search_components, "second_moment_fraction": round(eng.explained, 5),
+                   "rescore": f"top-{max(rk, a.topk)} exactly in FP32 on the full {D} pixels"}
+                  if eng.search_components else f"full {D} pixels")
+    pca_search = bool(eng.search_components)
     # time the projection kernel alone on the dtype it writes (with a PCA
     # search: the FP32 dictionary the basis and the rescoring come from)
     proj_dtype = DTYPE_F32 if eng.search_components else eng.dtype
@@ -195,7 +199,7 @@ def main():
     flops = 2.0 * a.batch * n_dict * search_dim
     achieved = flops / (sim_avg_ms / 1000) / 1e12
     peak = PEAKS[a.precision]
-    out_bytes = 4 if eng.search_components else {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[a.precision]
+    out_bytes = 4 if pca_search else {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[a.precision]
     proj_bytes = n_dict * (D * out_bytes + 16) + 2 * a.mp_size * a.mp_size * 4 + 12 * D
     proj_gbps = proj_bytes / (proj_kernel_ms / 1000) / 1e9
 
@@ -253,9 +257,7 @@ def main():
             "workload": f"DI {n_dict} m-3m dictionary x {a.batch}-pattern batch, {det}x{det} detector, top-{a.topk}",
             "dictionary": n_dict, "batch": a.batch, "detector": [det, det], "topk": a.topk,
             "precision": a.precision, "rescore_k": rk,
-            "search": ({"pca_components": eng.search_components, "second_moment_fraction": round(eng.explained, 5),
-                        "rescore": f"top-{max(rk, a.topk)} exactly in FP32 on the full {D} pixels"}
-                       if eng.search_components else f"full {D} pixels"),
+            "search": search_cfg,
         },
         "top1_agreement_vs_fp32": 1.0 if a.precision == "fp32" else agree,
         "agreement_sample": None if a.precision == "fp32" else min(a.agree_sample, a.batch),
@@ -521,7 +523,9 @@ def learned_form(a, mp, dico, dev):
     tlog = train_proposal_net(net, sim, a.learned_train, batch=512, lr=1e-3, warmup=200, log_every=500)
     log(f"learned: trained on {tlog.patterns} patterns in {tlog.seconds:.1f}s "
         f"({tlog.patterns / tlog.seconds:.0f}/s), loss {tlog.loss_curve[0][1]:.3f} -> {tlog.loss_curve[-1][1]:.4f}")
-    fb = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True)
+    sc = 0 if a.search_components == "0" else ("auto" if a.search_components == "auto" else int(a.search_components))
+    fb = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True, search_components=sc,
+                             explained_variance=a.explained_variance)
     binning, rescore_k = 4, 8
     idx = LearnedIndexer(net, mp, rays, 11, fallback=fb, threshold=0.0, binning=binning,
                          local_rays=geom.get_rays((binning, binning)).to(dev), rescore_k=rescore_k)
@@ -627,19 +631,28 @@ def c5_workload(a, dev):
     rays = geom.get_rays().to(dev)
     n_dict = dico.shape[0]
     torch.cuda.synchronize()
-    eng = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True)
+    sc = 0 if a.search_components == "0" else ("auto" if a.search_components == "auto" else int(a.search_components))
+    tb = time.perf_counter()
+    eng = E.DictionaryIndexer(mp, rays, dico, precision="fp8", rescore_k=16, dict_unit_norm=True,
+                              search_components=sc, explained_variance=a.explained_variance)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - tb
+    search_dim = eng.search_components if eng.search_components else D
     pe = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     pe[0].record()
-    kernels.project(mp, rays, dico, DTYPE_FP8_E4M3, unit_norm=True, out=eng.dictionary, quads=eng.quads)
+    if not eng.search_components:
+        kernels.project(mp, rays, dico, DTYPE_FP8_E4M3, unit_norm=True, out=eng.dictionary, quads=eng.quads)
     pe[1].record()
     kernels.project(mp, rays, dico, DTYPE_F32, unit_norm=True, out=eng.dictionary_f32, quads=eng.quads)
     pe[2].record()
     torch.cuda.synchronize()
-    proj8 = pe[0].elapsed_time(pe[1])
+    proj8 = None if eng.search_components else pe[0].elapsed_time(pe[1])
     proj32 = pe[1].elapsed_time(pe[2])
-    bytes8 = n_dict * (eng.dictionary.ld + 4 + 16) + 2 * 501 * 501 * 4 + 12 * D
+    bytes8 = None if eng.search_components else n_dict * (eng.dictionary.ld + 4 + 16) + 2 * 501 * 501 * 4 + 12 * D
     bytes32 = n_dict * (eng.dictionary_f32.ld * 4 + 16) + 2 * 501 * 501 * 4 + 12 * D
-    log(f"c5: dictionary {n_dict} x {D}: projection FP8 {proj8:.1f} ms, FP32 {proj32:.1f} ms")
+    log(f"c5: dictionary {n_dict} x {D}: built in {build_s:.1f} s, projection FP32 {proj32:.1f} ms"
+        + (f", FP8 {proj8:.1f} ms" if proj8 is not None else
+           f"; search on {eng.search_components} principal components ({eng.explained:.4f})"))
 
     sim = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.01, seed=20)
     net = ProposalNet((det, det), dev, laue_id=laue, compute="bf16", seed=0)
@@ -667,8 +680,7 @@ def c5_workload(a, dev):
         x, q_true = scan.batch(b)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        q8 = kernels.prep_patterns(x, DTYPE_FP8_E4M3, zero_mean=True, unit_norm=True)
-        q32 = kernels.prep_patterns(x, DTYPE_F32, zero_mean=True, unit_norm=True)
+        q8, q32 = eng.prepare_pair(x, unit_norm=True)
         val = torch.empty((b, k), device=dev)
         idx = torch.empty((b, k), dtype=torch.int32, device=dev)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -700,7 +712,7 @@ def c5_workload(a, dev):
     qs = torch.tensor([0.5, 0.9, 0.99, 0.999], dtype=torch.float64, device=err_di.device)
     full = [m for nb, m in sim_ms if nb == sim_ms[0][0]]
     sim_avg = sum(full) / len(full)
-    flops = 2.0 * sim_ms[0][0] * n_dict * D
+    flops = 2.0 * sim_ms[0][0] * n_dict * search_dim
     return {
         "metric": "c5: patterns indexed/s, 6/mmm 1M dictionary, 120x120, streamed scan",
         "workload": f"6/mmm {n_dict} dictionary x {a.c5_scan}-pattern scan in {chunk}-pattern chunks, {det}x{det}",
@@ -712,9 +724,12 @@ def c5_workload(a, dev):
                    "frac_within_1deg_of_truth": round(float((err_di <= 1.0).float().mean()), 5)},
         "similarity_kernel": {"avg_launch_ms": round(sim_avg, 2), "achieved_TFLOPs": round(flops / sim_avg / 1e9, 1),
                               "peak_TFLOPs": PEAKS["fp8"], "mfma_frac": round(flops / sim_avg / 1e9 / PEAKS["fp8"], 4)},
-        "projection": {"fp8_ms": round(proj8, 2), "fp8_GBps": round(bytes8 / proj8 / 1e6, 1),
+        "projection": {"fp8_ms": None if proj8 is None else round(proj8, 2),
+                       "fp8_GBps": None if proj8 is None else round(bytes8 / proj8 / 1e6, 1),
                        "fp32_ms": round(proj32, 2), "fp32_GBps": round(bytes32 / proj32 / 1e6, 1),
                        "hbm_peak_GBps": PEAKS["hbm_GBps"]},
+        "search": ({"pca_components": eng.search_components, "second_moment_fraction": round(eng.explained, 5),
+                    "dictionary_build_s": round(build_s, 2)} if eng.search_components else f"full {D} pixels"),
         "learned_proposal": {"patterns_per_s": round(a.c5_scan / t_le, 1), "fallback_rate": round(float(fb.float().mean()), 4),
                              "local_binning": c5_binning, "rescore_k": lidx.rescore_k,
                              "threshold": round(lidx.threshold, 5),

@@ -74,7 +74,7 @@ class DictionaryIndexer(_ResidentSearch):
     def __init__(self, master_pattern: torch.Tensor, rays: torch.Tensor, orientations: torch.Tensor,
                  precision: str = "fp32", rescore_k: int = 0, dict_unit_norm: bool = False,
                  renorm_dirs: bool = False, search_components: Union[int, str] = 0,
-                 explained_variance: float = 0.995, basis_sample: int = 1 << 18):
+                 explained_variance: float = 0.995, basis_sample: int = 1 << 18, rotate_basis: bool = True):
         if precision not in kernels.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(kernels.PRECISIONS)}")
         if search_components and rescore_k <= 0:
@@ -93,6 +93,7 @@ class DictionaryIndexer(_ResidentSearch):
         # sample in the projection (same values bit for bit)
         self.quads = kernels.mp_quad_table(self.mp)
         self.basis_t = None
+        self.rotate_basis = bool(rotate_basis)
         self.search_components = 0
         self.explained = 1.0
         if search_components:
@@ -112,7 +113,7 @@ class DictionaryIndexer(_ResidentSearch):
         exactly against the full FP32 dictionary. The second moment is
         accumulated on the matrix cores (ebsd_cov_update: centred aggregate,
         plus n mean mean^T) over up to `sample` dictionary rows and solved in
-        FP64 on the host. components "auto": the smallest multiple of 64 (at least the
+        FP64 (on the host up to D = 4096). components "auto": the smallest multiple of 64 (at least the
         search kernel's 4 K stages) whose leading eigenvalues hold
         `explained` of the trace; an int: that many."""
         from .pca_covmat import OnlineCovMatrix
@@ -126,9 +127,9 @@ class DictionaryIndexer(_ResidentSearch):
         n_obs = float(int(cov.obs.item()))
         mean = cov.mean.to(torch.float64)
         m2 = cov._full_aggregate().to(torch.float64) + n_obs * torch.outer(mean, mean)
-        # one FP64 solve per dictionary, on the host (LAPACK): ~1-2 s for
-        # D = 3600, and no device eigensolver in the indexing process
-        evals, evecs = torch.linalg.eigh(m2.cpu())
+        # one FP64 solve per dictionary: on the host (LAPACK, ~1-2 s) up to
+        # D = 4096, on the device above (the host solve grows as D^3)
+        evals, evecs = torch.linalg.eigh(m2.cpu() if d <= 4096 else m2)
         evals, evecs = evals.clamp_min(0.0).to(self.device), evecs.to(self.device)
         elem = kernels.TORCH_DTYPE[self.dtype].itemsize
         k_min = max(64, -(-256 // elem))  # >= 4 K stages of 64 bytes
@@ -140,7 +141,16 @@ class DictionaryIndexer(_ResidentSearch):
         k = min(max(-(-k // 64) * 64, k_min), d)
         self.search_components = k
         self.explained = float(evals[-k:].sum() / evals.sum())
-        basis = evecs[:, -k:].to(torch.float32)  # (d, k)
+        basis = evecs[:, -k:]  # (d, k) float64
+        if self.rotate_basis:
+            # a fixed random rotation of the subspace: the dot products are
+            # unchanged, the energy is spread evenly over the k coordinates,
+            # so the low-precision rounding errors of the coordinates average
+            # over k comparable terms instead of the few leading ones
+            g = torch.Generator().manual_seed(0x5eed)
+            rot, _ = torch.linalg.qr(torch.randn((k, k), generator=g, dtype=torch.float64))
+            basis = basis @ rot.to(basis.device)
+        basis = basis.to(torch.float32)
         bt = kernels.alloc_patterns(k, d, DTYPE_F32, self.device)
         bt.data.zero_()
         bt.data[:, :d] = basis.T
@@ -151,15 +161,20 @@ class DictionaryIndexer(_ResidentSearch):
         self.dictionary = kernels.prep_patterns(coef[:, :k], self.dtype, zero_mean=False)
         del coef
 
-    def prepare_pair(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None, zero_mean: bool = True):
-        """(search rows, FP32 rows for rescoring or None) of (N, P) patterns:
+    def prepare_pair(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None, zero_mean: bool = True,
+                     unit_norm: bool = False):
+        """(search rows, FP32 rows for rescoring or None) of (N, P) patterns
+        (unit_norm: also divide each row by its norm, for cosine scores):
         with a PCA-compressed search the search rows are the patterns'
         coordinates in the basis, in the search dtype."""
         if self.basis_t is None:
-            q = self.prepare(patterns, pix_idx, zero_mean)
-            qf = self.prepare(patterns, pix_idx, zero_mean, DTYPE_F32) if self.dictionary_f32 is not None else None
+            q = kernels.prep_patterns(patterns, self.dtype, zero_mean=zero_mean, unit_norm=unit_norm, pix_idx=pix_idx)
+            qf = None
+            if self.dictionary_f32 is not None:
+                qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, unit_norm=unit_norm,
+                                           pix_idx=pix_idx)
             return q, qf
-        qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, pix_idx=pix_idx)
+        qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, unit_norm=unit_norm, pix_idx=pix_idx)
         k = self.search_components
         qc = torch.empty((qf.rows, kernels.padded_width(k, DTYPE_F32)), dtype=torch.float32, device=self.device)
         kernels.gemm_nt(qf, self.basis_t, out=qc)

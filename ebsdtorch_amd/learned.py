@@ -544,10 +544,7 @@ class LearnedIndexer:
 
     def _full(self, p: torch.Tensor):
         fbk = self.fallback
-        q = kernels.prep_patterns(p, fbk.dtype, zero_mean=True, unit_norm=True)
-        q32 = None
-        if fbk.dictionary_f32 is not None:
-            q32 = kernels.prep_patterns(p, DTYPE_F32, zero_mean=True, unit_norm=True)
+        q, q32 = fbk.prepare_pair(p, unit_norm=True)
         val, idx = fbk.index(q, 1, queries_f32=q32)
         return val[:, 0], idx[:, 0]
 
