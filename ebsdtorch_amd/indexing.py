@@ -311,6 +311,7 @@ def dictionary_index_orientations(
     rescore_k: int = 0,
     dict_unit_norm: bool = False,
     generator: Optional[torch.Generator] = None,
+    search_components: Union[int, str] = 0,
 ) -> None:
     """Index experimental patterns against a projected dictionary per phase
     (indexing.py:14-211) and store the results in `experiment_patterns`:
@@ -327,6 +328,9 @@ def dictionary_index_orientations(
                              FP32 (precision form)
     dict_unit_norm           rank by cosine instead of the raw dot product
     generator                seeds the dictionary permutation
+    search_components        with rescore_k > 0: search on the dictionary's
+                             leading principal coordinates ("auto" or an
+                             int; 0 = full patterns), then rescore exactly
     """
     if isinstance(master_patterns, MasterPattern):
         master_patterns = [master_patterns]
@@ -353,7 +357,8 @@ def dictionary_index_orientations(
             rays = rays[flat]
             pix_idx = torch.nonzero(flat.to(device), as_tuple=False).flatten().to(torch.int32)
         engine = DictionaryIndexer(mp.pattern_for_binning(virtual_binning), rays.to(device), ori,
-                                   precision=precision, rescore_k=rescore_k, dict_unit_norm=dict_unit_norm)
+                                   precision=precision, rescore_k=rescore_k, dict_unit_norm=dict_unit_norm,
+                                   search_components=search_components)
         exp = experiment_patterns.get_patterns(torch.arange(n_exp, device=experiment_patterns.patterns.device),
                                                binning=virtual_binning)
         exp = exp.reshape(n_exp, -1).to(device, torch.float32)
@@ -365,8 +370,7 @@ def dictionary_index_orientations(
         # bound the prepared query buffer to ~4 GB per step
         step = max(1, min(n_exp, (4 << 30) // (4 * max(1, exp.shape[1]))))
         for s in range(0, n_exp, step):
-            q = engine.prepare(exp[s:s + step], pix_idx, zero_mean)
-            qf = engine.prepare(exp[s:s + step], pix_idx, zero_mean, DTYPE_F32) if engine.dictionary_f32 is not None else None
+            q, qf = engine.prepare_pair(exp[s:s + step], pix_idx, zero_mean)
             v, ix = engine.index(q, topk, qf)
             vals.append(v)
             idxs.append(ix)
