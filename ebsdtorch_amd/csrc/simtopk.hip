@@ -33,6 +33,8 @@
 // a second kernel merges the splits (and, optionally, the caller's running
 // state).
 
+#include <type_traits>
+
 #include "mfma_tile.h"
 
 namespace {
@@ -204,15 +206,15 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 //             E8M0 block scales (one byte per 32-row block, selected by
 //             op_sel), so scores come out scaled with no epilogue multiply
 //   top-k   : the two lane halves of a 32x32 tile hold the same 32 query
-//             columns but different rows. One v_permlane32_swap per pair of
-//             accumulator registers (column blocks s and s + 2) hands lane
-//             half h the rows of both halves of column block s + 2 h, so each
-//             lane keeps exactly two lists, one per query column it owns.
-//             A tile's scores are screened against the list thresholds; a
-//             32-score block that some lane may take is staged in the wave's
-//             LDS scratch and offered with a uniform index. At the end of a
-//             work unit the 2 lists of each query (one per dictionary wave)
-//             are merged through LDS into one partial list.
+//             columns but different rows; lane half h keeps the lists of
+//             column blocks 2 h and 2 h + 1, two per lane. A tile's scores
+//             are screened in place (per lane and 16 rows, the maximum
+//             against the threshold of the score's column); a 32-row block
+//             that some lane may take is written to the wave's LDS scratch
+//             in the rows of the lanes that own its columns, and each owner
+//             offers its marked scores in ascending row order. At the end of
+//             a work unit the 2 lists of each query (one per dictionary
+//             wave) are merged through LDS into one partial list.
 constexpr int BT_D = 192;                                 // dictionary rows per tile
 constexpr int BT_Q = 256;                                 // queries per tile
 constexpr int BT_WD = 96;                                 // dictionary rows per wave
@@ -221,8 +223,12 @@ constexpr int BT_STAGE = (BT_D + BT_Q) * BT_KB;           // 28 KiB
 constexpr int BT_NBUF = 4;
 constexpr int BT_SCALE_OFF = BT_NBUF * BT_STAGE;          // two tiles' dictionary scales
 constexpr int BT_THR_OFF = BT_SCALE_OFF + 2 * 256 * 4;    // thresholds [tile parity][wd][query]
-constexpr int BT_SCR_OFF = BT_THR_OFF + 2 * 2 * BT_Q * 4; // fold scratch, 8 KiB per wave
-constexpr int BT_LDS = BT_SCR_OFF + 4 * 32 * 64 * 4;
+// Scratch row of one lane in the fold: the 32 scores of a block for the query
+// column whose list the lane owns, in row order (36 floats, so the 16-byte
+// pieces written by 16 consecutive lanes start in 16 different bank groups).
+constexpr int BT_SCR_ROW = 36;
+constexpr int BT_SCR_OFF = BT_THR_OFF + 2 * 2 * BT_Q * 4; // fold scratch, 9 KiB per wave
+constexpr int BT_LDS = BT_SCR_OFF + 4 * 64 * BT_SCR_ROW * 4;
 constexpr int BT_THREADS = 256;
 // direct loads per wave per stage: 1-KiB data loads plus one scale load
 constexpr int BT_LOADS = BT_STAGE / 1024 / (BT_THREADS / 64) + 1;
@@ -299,46 +305,41 @@ __device__ __forceinline__ void bt_read(const unsigned char* __restrict__ buf, i
 // MFMAs IJ .. END - 1 (I = IJ / 4 dictionary block, J = IJ % 4 query block)
 // of K step S of one stage (FP8: one step of 64, BF16/FP16: two of 16).
 // FP8: sa / sb hold the E8M0 exponents of the lane's dictionary / query row
-// in byte i / j.
-template <int DT, int S, int IJ, int END>
+// in byte i / j. Z: the first step of a tile, which starts the accumulators
+// from zero (the MFMA's C operand is the constant 0).
+template <int DT, int S, int IJ, int END, bool Z>
 __device__ __forceinline__ void bt_mfma_step(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4]) {
   if constexpr (IJ < END) {
     constexpr int I = IJ / 4, J = IJ % 4;
+    const floatx16 c = Z ? floatx16{} : acc[I][J];
     if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
-      acc[I][J] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(f.a[I], f.b[J], acc[I][J], 0, 0, I, sa, J, sb);
+      acc[I][J] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(f.a[I], f.b[J], c, 0, 0, I, sa, J, sb);
     } else {
       const intx4 a4 = {f.a[I][4 * S], f.a[I][4 * S + 1], f.a[I][4 * S + 2], f.a[I][4 * S + 3]};
       const intx4 b4 = {f.b[J][4 * S], f.b[J][4 * S + 1], f.b[J][4 * S + 2], f.b[J][4 * S + 3]};
       if constexpr (DT == EBSD_DTYPE_BF16)
         acc[I][J] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a4),
-                                                            __builtin_bit_cast(bf16x8, b4), acc[I][J], 0, 0, 0);
+                                                            __builtin_bit_cast(bf16x8, b4), c, 0, 0, 0);
       else
         acc[I][J] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(halfx8, a4),
-                                                           __builtin_bit_cast(halfx8, b4), acc[I][J], 0, 0, 0);
+                                                           __builtin_bit_cast(halfx8, b4), c, 0, 0, 0);
     }
-    bt_mfma_step<DT, S, IJ + 1, END>(f, sa, sb, acc);
+    bt_mfma_step<DT, S, IJ + 1, END, Z>(f, sa, sb, acc);
   }
 }
 
 // All MFMAs of one stage, with hook(k) called after the k-th of the first 8
 // (k = 0 .. 7): the loads of a later stage go out in the shadow of the MFMAs.
-template <int DT, int IJ = 0, typename Hook>
+// Z: the tile's first stage (accumulators start from zero).
+template <int DT, bool Z, int IJ = 0, typename Hook>
 __device__ __forceinline__ void bt_mfma_hooked(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4], Hook&& hook) {
   if constexpr (IJ < 12) {
-    bt_mfma_step<DT, 0, IJ, IJ + 1>(f, sa, sb, acc);
+    bt_mfma_step<DT, 0, IJ, IJ + 1, Z>(f, sa, sb, acc);
     if constexpr (IJ < 8) hook(IJ);
-    bt_mfma_hooked<DT, IJ + 1>(f, sa, sb, acc, hook);
+    bt_mfma_hooked<DT, Z, IJ + 1>(f, sa, sb, acc, hook);
   } else if constexpr (DT != EBSD_DTYPE_FP8_E4M3) {
-    bt_mfma_step<DT, 1, 0, 12>(f, sa, sb, acc);
+    bt_mfma_step<DT, 1, 0, 12, false>(f, sa, sb, acc);
   }
-}
-
-// MFMAs of dictionary blocks [I0, I1) of one stage (both K steps for
-// BF16/FP16, step-major so consecutive MFMAs are independent).
-template <int DT, int I0, int I1>
-__device__ __forceinline__ void bt_mfma_rows(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4]) {
-  bt_mfma_step<DT, 0, 4 * I0, 4 * I1>(f, sa, sb, acc);
-  if constexpr (DT != EBSD_DTYPE_FP8_E4M3) bt_mfma_step<DT, 1, 4 * I0, 4 * I1>(f, sa, sb, acc);
 }
 
 __device__ __forceinline__ int e8m0_of(float s) { return (int)((__float_as_uint(s) >> 23) & 0xffu); }
@@ -352,70 +353,107 @@ __device__ __forceinline__ float bt_acc(float a) {
   return v;
 }
 
-// Zeroes an accumulator register in place (the asm result is pinned to an
-// AGPR, so the accumulators' home stays the AGPR file).
-__device__ __forceinline__ void bt_zero(floatx16& a) {
+// Maximum of three scores (v_max3_f32 as written: the scores are finite, so
+// the NaN quieting that fmaxf would add in front of every operand is moot).
+__device__ __forceinline__ float bt_max3(float a, float b, float c) {
+  float d;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
+// Largest of the 16 accumulator registers of one 32x32 tile in this lane.
+__device__ __forceinline__ float bt_max16(const floatx16& a) {
+  float v[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    float z;
-    asm volatile("v_accvgpr_write_b32 %0, 0" : "=a"(z));
-    a[r] = z;
+  for (int r = 0; r < 16; ++r) v[r] = bt_acc(a[r]);
+  const float m0 = bt_max3(v[0], v[1], v[2]), m1 = bt_max3(v[3], v[4], v[5]);
+  const float m2 = bt_max3(v[6], v[7], v[8]), m3 = bt_max3(v[9], v[10], v[11]);
+  const float m4 = bt_max3(v[12], v[13], v[14]);
+  return bt_max3(bt_max3(m0, m1, m2), m3, bt_max3(m4, v[15], v[15]));
+}
+
+// Insert (v, id) into a descending list if it beats the last entry, for
+// offers that arrive in ascending id order: an equal score then never
+// outranks a listed one (whose id is smaller), so one comparison per step
+// gives the order of topk_better.
+template <int K>
+__device__ __forceinline__ void topk_offer_ascending(float (&tv)[K], int (&ti)[K], float v, int id) {
+  if (!(v > tv[K - 1])) return;
+  tv[K - 1] = v;
+  ti[K - 1] = id;
+#pragma unroll
+  for (int j = K - 1; j > 0; --j) {
+    const bool b = tv[j] > tv[j - 1];
+    const float va = b ? tv[j] : tv[j - 1];
+    const float vb = b ? tv[j - 1] : tv[j];
+    const int ia = b ? ti[j] : ti[j - 1];
+    const int ib = b ? ti[j - 1] : ti[j];
+    tv[j - 1] = va;
+    tv[j] = vb;
+    ti[j - 1] = ia;
+    ti[j] = ib;
   }
 }
 
-// v_permlane32_swap of two accumulator registers: x keeps lanes 0-31 of `lo`
-// and gets lanes 0-31 of `hi` in lanes 32-63; y gets lanes 32-63 of `lo` in
-// lanes 0-31 and keeps lanes 32-63 of `hi`.
-__device__ __forceinline__ void bt_swap(float lo, float hi, float& x, float& y) {
-  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(bt_acc(lo)), __float_as_uint(bt_acc(hi)), false,
-                                                  false);
-  x = __uint_as_float(p[0]);
-  y = __uint_as_float(p[1]);
-}
-
-// Fold a finished 192 x 256 tile into the lane's two lists. dbase: index of
-// the wave's first dictionary row; n_valid: how many of its 96 rows exist;
-// thr[s]: a lower bound of the final K-th best score of list s's query;
-// scratch: the wave's 8 KiB of LDS, element e of lane l at float 64 e + l.
+// Fold a finished 192 x 256 tile into the lane's two lists. Lane (h, c)
+// (h = lane / 32, c = lane % 32) holds, of every 32 x 32 tile, column c and
+// rows 8 q + 4 h + rr (accumulator register 4 q + rr); its list s belongs to
+// column c of column block s + 2 h. dbase: index of the wave's first
+// dictionary row; n_valid: how many of its 96 rows exist; thr[s]: a lower
+// bound of the final K-th best score of list s's query; scratch: the wave's
+// 64 scratch rows. Offers reach each list in ascending dictionary index.
 template <int K>
 __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)[2][K], int (&ti)[2][K],
                                         const float (&thr)[2], int dbase, int n_valid, float* scratch, int lane) {
+  const int h = lane >> 5, c = lane & 31;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int s = 0; s < 2; ++s) {
+    // the thresholds of the two columns whose scores the lane holds: column c
+    // of block s (list s of lane c) and of block s + 2 (list s of lane 32 + c)
+    const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(thr[s]), __float_as_uint(thr[s]), false, false);
+    const float ta = __uint_as_float(t[0]), tb = __uint_as_float(t[1]);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      // element e = 2 r + x: accumulator register r of lane half x, i.e. row
-      // 8 (r / 4) + 4 x + r % 4 of the 32-row block i, for column block
-      // s + 2 h (h = this lane's half)
-      float mx = -INFINITY;
+    for (int i = 0; i < 3; ++i) {
+      const float ma = bt_max16(acc[i][s]);
+      const float mb = bt_max16(acc[i][s + 2]);
+      if (__builtin_amdgcn_ballot_w64(!(ma < ta) || !(mb < tb)) == 0) continue;
+      // Some lane may take a score: write the block's scores into the scratch
+      // rows of the lanes that own their columns (row 8 q + 4 h + rr at float
+      // 8 q + 4 h + rr), mark the ones that pass the owner's threshold, and let
+      // every owner walk its marked rows in ascending order (a divergent loop
+      // of max-popcount trips, not 32).
+      unsigned bits_a = 0, bits_b = 0;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        float x, y;
-        bt_swap(acc[i][s][r], acc[i][s + 2][r], x, y);
-        mx = fmaxf(mx, fmaxf(x, y));
+      for (int q = 0; q < 4; ++q) {
+        floatx4 va, vb;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          va[rr] = bt_acc(acc[i][s][4 * q + rr]);
+          vb[rr] = bt_acc(acc[i][s + 2][4 * q + rr]);
+        }
+        *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * c + 8 * q + 4 * h) = va;
+        *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * (32 + c) + 8 * q + 4 * h) = vb;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          bits_a |= (!(va[rr] < ta) ? 1u : 0u) << (8 * q + 4 * h + rr);
+          bits_b |= (!(vb[rr] < tb) ? 1u : 0u) << (8 * q + 4 * h + rr);
+        }
       }
-      if (__builtin_amdgcn_ballot_w64(!(mx < thr[s])) == 0) continue;
-      // Some lane may take a score: stage the block's 32 scores in LDS, mark
-      // the ones that pass the threshold, and let every lane walk only its
-      // own marked scores (a divergent loop of max-popcount trips, not 32).
-      unsigned m = 0;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        float x, y;
-        bt_swap(acc[i][s][r], acc[i][s + 2][r], x, y);
-        scratch[64 * (2 * r) + lane] = x;
-        scratch[64 * (2 * r + 1) + lane] = y;
-        m |= (!(x < thr[s]) ? 1u : 0u) << (2 * r);
-        m |= (!(y < thr[s]) ? 1u : 0u) << (2 * r + 1);
-      }
+      // the marks of the owned column from the partner lane (the other half)
+      const unsigned other = h ? bits_a : bits_b;
+      const auto o = __builtin_amdgcn_permlane32_swap(other, other, false, false);
+      unsigned m = (h ? bits_b : bits_a) | (h ? o[0] : o[1]);
+      // The partner's stores precede these loads in the wave's LDS order; the
+      // clobbers keep the compiler from moving memory accesses across.
+      asm volatile("" ::: "memory");
       while (m != 0) {
-        const int e = __builtin_ctz(m);
+        const int rho = __builtin_ctz(m);
         m &= m - 1;
-        const float v = scratch[64 * e + lane];
-        const int r = e >> 1;
-        const int row = 32 * i + ((r >> 2) << 3) + ((e & 1) << 2) + (r & 3);
-        if (row < n_valid) topk_offer<K>(tv[s], ti[s], v, dbase + row);
+        const float v = scratch[BT_SCR_ROW * lane + rho];
+        const int row = 32 * i + rho;
+        if (row < n_valid) topk_offer_ascending<K>(tv[s], ti[s], v, dbase + row);
       }
+      asm volatile("" ::: "memory");
     }
   }
 }
@@ -466,7 +504,7 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
   const int lchunk = (lane & 3) ^ ((lane >> 4) & 3);
   const float* sscale = reinterpret_cast<const float*>(smem + BT_SCALE_OFF);
   float* lthr = reinterpret_cast<float*>(smem + BT_THR_OFF);
-  float* scratch = reinterpret_cast<float*>(smem + BT_SCR_OFF) + wave * 32 * 64;
+  float* scratch = reinterpret_cast<float*>(smem + BT_SCR_OFF) + wave * 64 * BT_SCR_ROW;
 
   for (int grp = g_first; grp < sc.n_qgroups; grp += g_step) {
     const int qt = grp * sc.A + a;
@@ -489,11 +527,7 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     int ti[2][K];
     topk_init<K>(tv[0], ti[0]);
     topk_init<K>(tv[1], ti[1]);
-    floatx16 acc[3][4];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bt_zero(acc[i][j]);
+    floatx16 acc[3][4];  // written from zero by the first stage of every tile
 
     // Loads of stage g, in 8 parts: k < 3 dictionary rows 16 (wave + 4 k) ..,
     // k = 3 .. 6 query rows 16 (wave + 4 (k - 3)) .., k = 7 the dictionary
@@ -558,16 +592,19 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     }
     const int n_t = G / max(nk, 1);
     for (int tl = 0; tl < n_t; ++tl) {
-      for (int kt = 0; kt < nk; ++kt) {
-        const int g = tl * nk + kt;
+      // stage g of the tile; first: the tile's first stage
+      auto stage = [&](int g, auto first) {
         bt_wait_barrier<2 * BT_LOADS>();
         BtFrag f;
         bt_read<DT>(smem + (g % BT_NBUF) * BT_STAGE, wd, wq, lane, f);
         __builtin_amdgcn_sched_barrier(0);
         // the MFMAs start as their operands come in; the loads of stage
         // g + 3 (into the buffer of stage g - 1) go out between them
-        bt_mfma_hooked<DT>(f, sa, sb, acc, [&](int k) { issue_part(g + BT_NBUF - 1, k); });
-      }
+        bt_mfma_hooked<DT, decltype(first)::value>(f, sa, sb, acc,
+                                                   [&](int k) { issue_part(g + BT_NBUF - 1, k); });
+      };
+      stage(tl * nk, std::true_type{});
+      for (int kt = 1; kt < nk; ++kt) stage(tl * nk + kt, std::false_type{});
       // A score below the K-th best of either of the query's 2 lists cannot
       // reach its final top-K, so offers are filtered with the larger list
       // threshold known: this wave's now, the other dictionary wave's as of
@@ -586,10 +623,6 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
       bt_fold<K>(acc, tv, ti, thr, index_base + d0w, N - d0w, scratch, lane);
       // the next tile's scales came with this tile's stages
       if (tl + 1 < n_t) sa = dict_scales(t_begin + tl + 1);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) bt_zero(acc[i][j]);
       // no LDS operation of the fold stays in flight into the next tile's
       // stages (lgkmcnt(0); vmcnt and expcnt unconstrained), so the waits
       // there only count the stage's own reads
