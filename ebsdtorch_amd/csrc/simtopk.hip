@@ -33,6 +33,7 @@
 // a second kernel merges the splits (and, optionally, the caller's running
 // state).
 
+#include <algorithm>
 #include <type_traits>
 
 #include "mfma_tile.h"
@@ -222,12 +223,12 @@ constexpr int BT_KB = 64;                                 // bytes of K per row 
 constexpr int BT_STAGE = (BT_D + BT_Q) * BT_KB;           // 28 KiB
 constexpr int BT_NBUF = 4;
 constexpr int BT_SCALE_OFF = BT_NBUF * BT_STAGE;          // two tiles' dictionary scales
-constexpr int BT_THR_OFF = BT_SCALE_OFF + 2 * 256 * 4;    // thresholds [tile parity][wd][query]
+constexpr int BT_THR_OFF = BT_SCALE_OFF + 2 * 256 * 4;    // list bounds [tile parity][wd][2][query]
 // Scratch row of one lane in the fold: the 32 scores of a block for the query
 // column whose list the lane owns, in row order (36 floats, so the 16-byte
 // pieces written by 16 consecutive lanes start in 16 different bank groups).
 constexpr int BT_SCR_ROW = 36;
-constexpr int BT_SCR_OFF = BT_THR_OFF + 2 * 2 * BT_Q * 4; // fold scratch, 9 KiB per wave
+constexpr int BT_SCR_OFF = BT_THR_OFF + 2 * 2 * 2 * BT_Q * 4; // fold scratch, 9 KiB per wave
 constexpr int BT_LDS = BT_SCR_OFF + 4 * 64 * BT_SCR_ROW * 4;
 constexpr int BT_THREADS = 256;
 // direct loads per wave per stage: 1-KiB data loads plus one scale load
@@ -395,6 +396,12 @@ __device__ __forceinline__ void topk_offer_ascending(float (&tv)[K], int (&ti)[K
   }
 }
 
+// Bit j = 4 q + rr of a 16-bit mask to bit 8 q + rr (q, rr < 4).
+__device__ __forceinline__ unsigned bt_spread_rows(unsigned p) {
+  const unsigned y = (p | (p << 8)) & 0x00ff00ffu;
+  return (y | (y << 4)) & 0x0f0f0f0fu;
+}
+
 // Fold a finished 192 x 256 tile into the lane's two lists. Lane (h, c)
 // (h = lane / 32, c = lane % 32) holds, of every 32 x 32 tile, column c and
 // rows 8 q + 4 h + rr (accumulator register 4 q + rr); its list s belongs to
@@ -422,9 +429,12 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)
       // 8 q + 4 h + rr), mark the ones that pass the owner's threshold, and let
       // every owner walk its marked rows in ascending order (a divergent loop
       // of max-popcount trips, not 32).
-      unsigned bits_a = 0, bits_b = 0;
+      // fail bit j = 4 q + rr: the score of row 8 q + 4 h + rr is below the
+      // owner's threshold, i.e. the sign of score - threshold (+0 for equal
+      // finite values), shifted in from j = 15 down to j = 0
+      unsigned fail_a = 0, fail_b = 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 3; q >= 0; --q) {
         floatx4 va, vb;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
@@ -434,11 +444,13 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)
         *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * c + 8 * q + 4 * h) = va;
         *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * (32 + c) + 8 * q + 4 * h) = vb;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          bits_a |= (!(va[rr] < ta) ? 1u : 0u) << (8 * q + 4 * h + rr);
-          bits_b |= (!(vb[rr] < tb) ? 1u : 0u) << (8 * q + 4 * h + rr);
+        for (int rr = 3; rr >= 0; --rr) {
+          fail_a = __builtin_amdgcn_alignbit(fail_a, __float_as_uint(va[rr] - ta), 31);
+          fail_b = __builtin_amdgcn_alignbit(fail_b, __float_as_uint(vb[rr] - tb), 31);
         }
       }
+      const unsigned bits_a = bt_spread_rows(~fail_a & 0xffffu) << (4 * h);
+      const unsigned bits_b = bt_spread_rows(~fail_b & 0xffffu) << (4 * h);
       // the marks of the owned column from the partner lane (the other half)
       const unsigned other = h ? bits_a : bits_b;
       const auto o = __builtin_amdgcn_permlane32_swap(other, other, false, false);
@@ -605,19 +617,27 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
       };
       stage(tl * nk, std::true_type{});
       for (int kt = 1; kt < nk; ++kt) stage(tl * nk + kt, std::false_type{});
-      // A score below the K-th best of either of the query's 2 lists cannot
-      // reach its final top-K, so offers are filtered with the larger list
-      // threshold known: this wave's now, the other dictionary wave's as of
-      // the previous tile (thresholds only rise).
+      // A score below the K-th best over the query's 2 lists (one per
+      // dictionary wave, disjoint rows) cannot reach its final top-K. Lower
+      // bounds of that: either list's K-th best, and the smaller of the two
+      // lists' ((K + 1) / 2)-th bests (together at least K scores at least as
+      // good). The other wave's values are those after the previous tile
+      // (they only rise); offers are filtered with the largest bound.
       float thr[2];
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const int ql = 128 * wq + 32 * (s + 2 * h) + r32;
         const float own = tv[s][K - 1];
+        const float own_mid = tv[s][(K + 1) / 2 - 1];
         float th = own;
-        if (tl > 0) th = fmaxf(th, lthr[((tl - 1) & 1) * 2 * BT_Q + (wd ^ 1) * BT_Q + ql]);
+        if (tl > 0) {
+          const float* o = lthr + ((((tl - 1) & 1) * 2 + (wd ^ 1)) * 2) * BT_Q + ql;
+          th = fmaxf(th, fmaxf(o[0], fminf(own_mid, o[BT_Q])));
+        }
         thr[s] = th;
-        lthr[(tl & 1) * 2 * BT_Q + wd * BT_Q + ql] = own;
+        float* w = lthr + (((tl & 1) * 2 + wd) * 2) * BT_Q + ql;
+        w[0] = own;
+        w[BT_Q] = own_mid;
       }
       const int d0w = (t_begin + tl) * BT_D + BT_WD * wd;
       bt_fold<K>(acc, tv, ti, thr, index_base + d0w, N - d0w, scratch, lane);
@@ -758,8 +778,15 @@ constexpr int kXcds = 8;
 constexpr int kSlotsPerXcd = 64;
 // Query tiles per XCD group of the 192 x 256 kernel (A of plan_groups): 16
 // tiles x 2 dictionary parts; each dictionary tile is then streamed by 16 CUs
-// of an XCD at once, which measured fastest (A = 2, 4, 8 were 5-20% slower).
+// of an XCD at once, which measured fastest (A = 2, 4, 8 were 5-20% slower;
+// A = 32 took 16% longer with 3,600-byte rows, as long with 448).
+// Rows of at most 256 bytes take 32 tiles x 1 part: the group's queries
+// (2 MiB) still fit in the XCD's 4 MiB L2, and one part per query tile
+// halves the fold's early, low-threshold work (28.1 -> 24.1 ms for
+// 65,536 x 995,277 at 256 bytes).
 constexpr int kBtGroupTiles = 16;
+constexpr int kBtGroupTilesShort = 32;
+constexpr int kBtShortRowBytes = 256;
 
 void plan_groups(SimSched& s, int slots, int a_max, int n_tiles, int& n_parts) {
   // Largest A <= a_max that still gives every XCD a query group.
@@ -779,7 +806,8 @@ SimPlan plan_sim(int m, int n, int topk, int dtype, int ld_bytes) {
   p.bt = dtype != EBSD_DTYPE_F32 && n >= BT_D && ld_bytes / BT_KB >= BT_MIN_STAGES;
   if (p.bt) {
     s.n_qtiles = (m + BT_Q - 1) / BT_Q;
-    plan_groups(s, BT_SLOTS_PER_XCD, kBtGroupTiles, (n + BT_D - 1) / BT_D, p.n_parts);
+    plan_groups(s, BT_SLOTS_PER_XCD, ld_bytes <= kBtShortRowBytes ? kBtGroupTilesShort : kBtGroupTiles,
+                (n + BT_D - 1) / BT_D, p.n_parts);
     p.grid = kXcds * BT_SLOTS_PER_XCD;
   } else {
     s.n_qtiles = (m + TILE - 1) / TILE;
@@ -824,10 +852,12 @@ extern "C" int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t
   if (m < 0 || n < 1) return EBSD_ERR_SHAPE;
   if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
   if (elem_bytes(dtype) == 0) return EBSD_ERR_DTYPE;
-  // the plan also depends on the row length; cover both kernels
+  // the plan also depends on the row length; cover both kernels and both
+  // groupings of the 192 x 256 one
   const size_t b1 = workspace_bytes(plan_sim(m, n, topk, EBSD_DTYPE_F32, STAGE_BYTES), m);
-  const size_t b2 = workspace_bytes(plan_sim(m, n, topk, dtype, BT_MIN_STAGES * BT_KB), m);
-  *bytes = b1 > b2 ? b1 : b2;
+  const size_t b2 = workspace_bytes(plan_sim(m, n, topk, dtype, kBtShortRowBytes), m);
+  const size_t b3 = workspace_bytes(plan_sim(m, n, topk, dtype, kBtShortRowBytes + BT_KB), m);
+  *bytes = std::max(b1, std::max(b2, b3));
   return EBSD_OK;
 }
 
