@@ -458,12 +458,18 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)
       // The partner's stores precede these loads in the wave's LDS order; the
       // clobbers keep the compiler from moving memory accesses across.
       asm volatile("" ::: "memory");
+      // the score of the next marked row is read ahead of the current row's
+      // insertion (for an empty rest, row 31's: a harmless in-bounds read)
+      int rho = __builtin_ctz(m | 0x80000000u);
+      float v = scratch[BT_SCR_ROW * lane + rho];
       while (m != 0) {
-        const int rho = __builtin_ctz(m);
         m &= m - 1;
-        const float v = scratch[BT_SCR_ROW * lane + rho];
+        const int rho_next = __builtin_ctz(m | 0x80000000u);
+        const float v_next = scratch[BT_SCR_ROW * lane + rho_next];
         const int row = 32 * i + rho;
         if (row < n_valid) topk_offer_ascending<K>(tv[s], ti[s], v, dbase + row);
+        rho = rho_next;
+        v = v_next;
       }
       asm volatile("" ::: "memory");
     }
