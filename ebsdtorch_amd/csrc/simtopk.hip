@@ -735,6 +735,13 @@ __global__ __launch_bounds__(256) void topk_merge_kernel(
 
 // FP32 re-scoring of candidate lists: one wave per query computes the exact
 // FP32 dot product with each candidate row, then keeps the best `topk`.
+// Candidates go in groups of kRescoreGroup, so each load of the query feeds
+// that many products and as many dictionary rows are fetched at once (16
+// candidates of 3,600 pixels for 65,536 queries: 4.07 ms one at a time,
+// 3.04 ms in groups of 4, 2.87 ms of 8, 3.05 ms of 16); a slot that holds no
+// dictionary row (or pads the last group) reads row 0 and is dropped.
+constexpr int kRescoreGroup = 8;
+
 template <int C>
 __global__ __launch_bounds__(256) void rescore_kernel(
     const float* __restrict__ Q, int M, const float* __restrict__ Dm, int N, int ld,
@@ -748,18 +755,30 @@ __global__ __launch_bounds__(256) void rescore_kernel(
   float tv[C];
   int ti[C];
   topk_init<C>(tv, ti);
-  for (int j = 0; j < n_cand; ++j) {
-    const int id = cand[(size_t)q * n_cand + j];  // wave-uniform
-    if (id < 0 || id >= N) continue;  // empty slot or not a dictionary row
-    const floatx4* dr = reinterpret_cast<const floatx4*>(Dm + (size_t)id * ld);
-    float acc = 0.0f;
+  for (int j0 = 0; j0 < n_cand; j0 += kRescoreGroup) {
+    int id[kRescoreGroup];
+    bool ok[kRescoreGroup];
+    const floatx4* dr[kRescoreGroup];
+#pragma unroll
+    for (int u = 0; u < kRescoreGroup; ++u) {
+      id[u] = j0 + u < n_cand ? cand[(size_t)q * n_cand + j0 + u] : -1;  // wave-uniform
+      ok[u] = id[u] >= 0 && id[u] < N;  // else an empty slot or not a dictionary row
+      dr[u] = reinterpret_cast<const floatx4*>(Dm + (size_t)(ok[u] ? id[u] : 0) * ld);
+    }
+    float acc[kRescoreGroup] = {};
     for (int c = lane; c < n4; c += 64) {
       const floatx4 a = qr[c];
-      const floatx4 b = dr[c];
-      acc += a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+#pragma unroll
+      for (int u = 0; u < kRescoreGroup; ++u) {
+        const floatx4 b = dr[u][c];
+        acc[u] += a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+      }
     }
-    acc = wave_sum(acc);
-    topk_offer<C>(tv, ti, acc, id);
+#pragma unroll
+    for (int u = 0; u < kRescoreGroup; ++u) {
+      const float v = wave_sum(acc[u]);
+      if (ok[u]) topk_offer<C>(tv, ti, v, id[u]);
+    }
   }
   if (lane == 0) {
     for (int j = 0; j < topk; ++j) {
