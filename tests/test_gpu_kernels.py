@@ -232,6 +232,50 @@ def test_sim_topk_large_tile_kernel(dtype, m, n, d, k):
     _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
 
 
+@pytest.mark.parametrize("dtype,d,k", [(DTYPE_FP8_E4M3, 256, 16), (DTYPE_BF16, 128, 16), (DTYPE_FP8_E4M3, 384, 16),
+                                       (DTYPE_FP8_E4M3, 256, 1), (DTYPE_F16, 256, 4)])
+def test_sim_topk_large_tile_exact_ties_over_many_tiles(dtype, d, k):
+    """A dictionary of identical row pairs (2j, 2j + 1), so every score ties
+    exactly with its partner, and 65,536 queries, so that each work unit
+    streams all 469 dictionary tiles (rows of at most 256 bytes: one part per
+    query tile) or half of them (longer rows). Equal scores rank by smaller
+    index: a pair in the list is adjacent with the smaller index first, and
+    a pair cut by the K-th place keeps only its smaller index. Values equal
+    the float64 products of the stored values (checked on 1,024 queries)."""
+    _say(f"sim_topk exact ties dtype={dtype} d={d} k={k}")
+    m, n = 65536, 2 * 45000
+    g = torch.Generator(device=DEV).manual_seed(d + k + dtype)
+    q = torch.randn(m, d, generator=g, device=DEV)
+    half = torch.randn(n // 2, d, generator=g, device=DEV)
+    dd = half.repeat_interleave(2, dim=0)
+    Q = kernels.prep_patterns(q, dtype, zero_mean=False)
+    D = kernels.prep_patterns(dd, dtype, zero_mean=False)
+    val = torch.empty(m, k, device=DEV)
+    idx = torch.empty(m, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, val, idx, merge=False)
+    sel = torch.arange(0, m, 64, device=DEV)
+    full = Q.to_float()[sel].double() @ D.to_float().double().T
+    ref_v, _ = torch.topk(full, k, dim=1)
+    tol = 2e-5 * ref_v.abs().max().item()
+    assert torch.allclose(val[sel].double(), ref_v, atol=tol)
+    ix = idx[sel].long()
+    assert torch.allclose(full.gather(1, ix), ref_v, atol=tol)
+    assert torch.all(val[:, 1:] <= val[:, :-1])
+    ix_c = idx.long().cpu()
+    for r in range(m):
+        row = ix_c[r].tolist()
+        present = set(row)
+        for pos, i in enumerate(row):
+            partner = i ^ 1
+            if partner in present:
+                # both in: adjacent, smaller index first
+                j = row.index(partner)
+                assert abs(j - pos) == 1 and (row[min(j, pos)] == min(i, partner)), (r, row)
+            else:
+                # cut by the K-th place: only the smaller index may stay
+                assert i % 2 == 0, (r, row)
+
+
 @pytest.mark.parametrize("dtype", [DTYPE_BF16, DTYPE_F16, DTYPE_FP8_E4M3])
 def test_sim_topk_low_precision_streaming_mixes_kernels(dtype):
     """Chunks below 192 rows take the 128 x 128 kernel, larger ones the
