@@ -413,40 +413,68 @@ template <int K>
 __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)[2][K], int (&ti)[2][K],
                                         const float (&thr)[2], int dbase, int n_valid, float* scratch, int lane) {
   const int h = lane >> 5, c = lane & 31;
+  // the thresholds of the two columns whose scores the lane holds: column c
+  // of block s (list s of lane c) and of block s + 2 (list s of lane 32 + c)
+  float ta[2], tb[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    // the thresholds of the two columns whose scores the lane holds: column c
-    // of block s (list s of lane c) and of block s + 2 (list s of lane 32 + c)
     const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(thr[s]), __float_as_uint(thr[s]), false, false);
-    const float ta = __uint_as_float(t[0]), tb = __uint_as_float(t[1]);
+    ta[s] = __uint_as_float(t[0]);
+    tb[s] = __uint_as_float(t[1]);
+  }
+  // Screen every 32-row block first, with no branch in between: bit
+  // 2 (3 s + i) (resp. + 1) is set when some lane may take a score of column
+  // block s (resp. s + 2) in row block i.
+  unsigned hits = 0;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const float ma = bt_max16(acc[i][s]);
       const float mb = bt_max16(acc[i][s + 2]);
-      if (__builtin_amdgcn_ballot_w64(!(ma < ta) || !(mb < tb)) == 0) continue;
-      // Some lane may take a score: write the block's scores into the scratch
-      // rows of the lanes that own their columns (row 8 q + 4 h + rr at float
-      // 8 q + 4 h + rr), mark the ones that pass the owner's threshold, and let
-      // every owner walk its marked rows in ascending order (a divergent loop
-      // of max-popcount trips, not 32).
+      if (__builtin_amdgcn_ballot_w64(!(ma < ta[s])) != 0) hits |= 1u << (2 * (3 * s + i));
+      if (__builtin_amdgcn_ballot_w64(!(mb < tb[s])) != 0) hits |= 2u << (2 * (3 * s + i));
+    }
+  }
+  if (hits == 0) return;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const unsigned hb = (hits >> (2 * (3 * s + i))) & 3u;
+      if (hb == 0) continue;
+      // Some lane may take a score: write the scores of the column half (or
+      // halves) concerned into the scratch rows of the lanes that own their
+      // columns (row 8 q + 4 h + rr at float 8 q + 4 h + rr), mark the ones
+      // that pass the owner's threshold, and let every owner walk its marked
+      // rows in ascending order (a divergent loop of max-popcount trips).
       // fail bit j = 4 q + rr: the score of row 8 q + 4 h + rr is below the
       // owner's threshold, i.e. the sign of score - threshold (+0 for equal
-      // finite values), shifted in from j = 15 down to j = 0
-      unsigned fail_a = 0, fail_b = 0;
+      // finite values), shifted in from j = 15 down to j = 0; a half that is
+      // not staged fails everywhere
+      unsigned fail_a = 0xffffu, fail_b = 0xffffu;
+      if (hb & 1u) {
 #pragma unroll
-      for (int q = 3; q >= 0; --q) {
-        floatx4 va, vb;
+        for (int q = 3; q >= 0; --q) {
+          floatx4 va;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          va[rr] = bt_acc(acc[i][s][4 * q + rr]);
-          vb[rr] = bt_acc(acc[i][s + 2][4 * q + rr]);
+          for (int rr = 0; rr < 4; ++rr) va[rr] = bt_acc(acc[i][s][4 * q + rr]);
+          *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * c + 8 * q + 4 * h) = va;
+#pragma unroll
+          for (int rr = 3; rr >= 0; --rr)
+            fail_a = __builtin_amdgcn_alignbit(fail_a, __float_as_uint(va[rr] - ta[s]), 31);
         }
-        *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * c + 8 * q + 4 * h) = va;
-        *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * (32 + c) + 8 * q + 4 * h) = vb;
+      }
+      if (hb & 2u) {
 #pragma unroll
-        for (int rr = 3; rr >= 0; --rr) {
-          fail_a = __builtin_amdgcn_alignbit(fail_a, __float_as_uint(va[rr] - ta), 31);
-          fail_b = __builtin_amdgcn_alignbit(fail_b, __float_as_uint(vb[rr] - tb), 31);
+        for (int q = 3; q >= 0; --q) {
+          floatx4 vb;
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) vb[rr] = bt_acc(acc[i][s + 2][4 * q + rr]);
+          *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * (32 + c) + 8 * q + 4 * h) = vb;
+#pragma unroll
+          for (int rr = 3; rr >= 0; --rr)
+            fail_b = __builtin_amdgcn_alignbit(fail_b, __float_as_uint(vb[rr] - tb[s]), 31);
         }
       }
       const unsigned bits_a = bt_spread_rows(~fail_a & 0xffffu) << (4 * h);
