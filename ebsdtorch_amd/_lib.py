@@ -74,6 +74,11 @@ _SIGNATURES = {
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int,
          _c_void_p, _c_int, _c_void_p],
     ),
+    "ebsd_prep_patterns_pair": (
+        _c_int,
+        [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_int,
+         _c_void_p],
+    ),
     "ebsd_rescore": (
         _c_int,
         [_c_void_p, _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p]),

@@ -189,6 +189,34 @@ __global__ __launch_bounds__(256) void prep_kernel(
   finish_row<ITEMS, OutT>(vals, n_pix, flags, out, ld, out_scale, n, red);
 }
 
+// prep_kernel with a second output: the finished FP32 row and the same
+// values converted to Out2 (BF16 / FP16), from one read of the patterns.
+template <int ITEMS, typename Out2>
+__global__ __launch_bounds__(256) void prep_pair_kernel(
+    const float* __restrict__ pats, int ld_in, const int* __restrict__ pix_idx, int n_pix,
+    float* __restrict__ out, int ld, Out2* __restrict__ out2, int ld2, int flags) {
+  __shared__ float red[4];
+  const size_t n = blockIdx.x;
+  const float* row = pats + n * (size_t)ld_in;
+  float vals[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const int d = threadIdx.x + i * 256;
+    float v = 0.0f;
+    if (d < n_pix) v = row[pix_idx ? pix_idx[d] : d];
+    vals[i] = v;
+  }
+  // finish_row leaves the finished values in vals (FP32 output: no scale)
+  finish_row<ITEMS, float>(vals, n_pix, flags, out, ld, nullptr, n, red);
+  Out2* orow = out2 + n * (size_t)ld2;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const int d = threadIdx.x + i * 256;
+    if (d < ld2) orow[d] = to_out<Out2>(d < n_pix ? vals[i] : 0.0f);
+  }
+  for (int d = ITEMS * 256 + threadIdx.x; d < ld2; d += 256) orow[d] = to_out<Out2>(0.0f);
+}
+
 // Sample the master pattern at arbitrary sphere directions (one per thread).
 __global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restrict__ mp, int H, int W,
                                                           const float* __restrict__ dirs, int64_t n,
@@ -223,6 +251,28 @@ int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, 
   EBSD_PROJ_CASE(32)
   EBSD_PROJ_CASE(64)
 #undef EBSD_PROJ_CASE
+  return EBSD_ERR_SHAPE;
+}
+
+template <typename Out2>
+int launch_prep_pair(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix, float* out, int ld,
+                     void* out2, int ld2, int flags, hipStream_t st) {
+  const dim3 grid(n), block(256);
+  Out2* o2 = static_cast<Out2*>(out2);
+  const int items = (n_pix + 255) / 256;
+#define EBSD_PREP_PAIR_CASE(I)                                                                          \
+  if (items <= I) {                                                                                     \
+    hipLaunchKernelGGL((prep_pair_kernel<I, Out2>), grid, block, 0, st, pats, ld_in, pix_idx, n_pix, out, ld, \
+                       o2, ld2, flags);                                                                 \
+    EBSD_LAUNCH_CHECK();                                                                                \
+    return EBSD_OK;                                                                                     \
+  }
+  EBSD_PREP_PAIR_CASE(4)
+  EBSD_PREP_PAIR_CASE(8)
+  EBSD_PREP_PAIR_CASE(16)
+  EBSD_PREP_PAIR_CASE(32)
+  EBSD_PREP_PAIR_CASE(64)
+#undef EBSD_PREP_PAIR_CASE
   return EBSD_ERR_SHAPE;
 }
 
@@ -388,4 +438,22 @@ extern "C" int ebsd_prep_patterns(const float* pats, int n, int ld_in, const int
       return launch_prep<uint8_t>(pats, n, ld_in, pix_idx, n_pix, out, ld_out, out_scale, flags, st);
   }
   return EBSD_ERR_DTYPE;
+}
+
+extern "C" int ebsd_prep_patterns_pair(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix,
+                                       float* out_f32, int ld_f32, void* out2, int out2_dtype, int ld2, int flags,
+                                       void* stream) {
+  if (!pats || !out_f32 || !out2) return EBSD_ERR_NULL;
+  if (n < 0 || ld_in < 1) return EBSD_ERR_SHAPE;
+  if (!pix_idx && n_pix > ld_in) return EBSD_ERR_SHAPE;
+  if (out2_dtype != EBSD_DTYPE_BF16 && out2_dtype != EBSD_DTYPE_F16) return EBSD_ERR_DTYPE;
+  int chk = check_ld(EBSD_DTYPE_F32, n_pix, ld_f32);
+  if (chk) return chk;
+  chk = check_ld(out2_dtype, n_pix, ld2);
+  if (chk) return chk;
+  if (n == 0) return EBSD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (out2_dtype == EBSD_DTYPE_BF16)
+    return launch_prep_pair<__hip_bfloat16>(pats, n, ld_in, pix_idx, n_pix, out_f32, ld_f32, out2, ld2, flags, st);
+  return launch_prep_pair<__half>(pats, n, ld_in, pix_idx, n_pix, out_f32, ld_f32, out2, ld2, flags, st);
 }

@@ -20,7 +20,7 @@ from typing import List, Optional, Sequence, Union
 import torch
 
 from . import kernels
-from ._lib import DTYPE_F32, DTYPE_FP8_E4M3
+from ._lib import DTYPE_BF16, DTYPE_F32, DTYPE_FP8_E4M3
 from .experiment_pats import ExperimentPatterns
 from .geometry import EBSDGeometry
 from .knn import ChunkedKNN
@@ -93,6 +93,7 @@ class DictionaryIndexer(_ResidentSearch):
         # sample in the projection (same values bit for bit)
         self.quads = kernels.mp_quad_table(self.mp)
         self.basis_t = None
+        self.basis_t_bf16 = None
         self.rotate_basis = bool(rotate_basis)
         self.search_components = 0
         self.explained = 1.0
@@ -155,6 +156,11 @@ class DictionaryIndexer(_ResidentSearch):
         bt.data.zero_()
         bt.data[:, :d] = basis.T
         self.basis_t = bt
+        # FP8 search: the query coordinates are rounded to FP8 right after the
+        # projection, so it runs on BF16 operands (half the bytes, the BF16
+        # matrix cores); the dictionary coordinates above stay FP32-exact
+        self.basis_t_bf16 = (kernels.prep_patterns(basis.T.contiguous(), DTYPE_BF16, zero_mean=False)
+                             if self.dtype == DTYPE_FP8_E4M3 else None)
         coef = torch.empty((full.rows, kernels.padded_width(k, DTYPE_F32)), dtype=torch.float32, device=self.device)
         for s in range(0, full.rows, chunk):
             kernels.gemm_nt(full.narrow(s, min(chunk, full.rows - s)), bt, out=coef[s:s + chunk])
@@ -174,10 +180,18 @@ class DictionaryIndexer(_ResidentSearch):
                 qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, unit_norm=unit_norm,
                                            pix_idx=pix_idx)
             return q, qf
-        qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, unit_norm=unit_norm, pix_idx=pix_idx)
         k = self.search_components
-        qc = torch.empty((qf.rows, kernels.padded_width(k, DTYPE_F32)), dtype=torch.float32, device=self.device)
-        kernels.gemm_nt(qf, self.basis_t, out=qc)
+        if self.basis_t_bf16 is not None:
+            qf, qb = kernels.prep_patterns_pair(patterns, DTYPE_BF16, zero_mean=zero_mean, unit_norm=unit_norm,
+                                                pix_idx=pix_idx)
+            qc = torch.empty((qf.rows, kernels.padded_width(k, DTYPE_F32)), dtype=torch.float32, device=self.device)
+            kernels.gemm_nt(qb, self.basis_t_bf16, out=qc)
+            del qb
+        else:
+            qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, unit_norm=unit_norm,
+                                       pix_idx=pix_idx)
+            qc = torch.empty((qf.rows, kernels.padded_width(k, DTYPE_F32)), dtype=torch.float32, device=self.device)
+            kernels.gemm_nt(qf, self.basis_t, out=qc)
         return kernels.prep_patterns(qc[:, :k], self.dtype, zero_mean=False), qf
 
     def _project(self, dtype: int) -> kernels.PatternMatrix:

@@ -132,6 +132,38 @@ def project(master_pattern: torch.Tensor, rays: torch.Tensor, quats: torch.Tenso
     return out
 
 
+def prep_patterns_pair(pats: torch.Tensor, dtype2: int, zero_mean: bool = True, unit_norm: bool = False,
+                       pix_idx: Optional[torch.Tensor] = None):
+    """prep_patterns to float32 and to dtype2 (BF16 / FP16) in one pass
+    (ebsd_prep_patterns_pair): (float32 rows, dtype2 rows), the second equal
+    bit for bit to prep_patterns(pats, dtype2, ...)."""
+    require_device(pats, "patterns")
+    if dtype2 not in (DTYPE_BF16, DTYPE_F16):
+        raise ValueError("the second output is BF16 or FP16")
+    if pats.dtype != torch.float32:
+        pats = pats.float()
+    if pats.dim() != 2:
+        raise ValueError("patterns must be (N, P)")
+    if pats.stride(1) != 1:
+        pats = pats.contiguous()
+    n, p = pats.shape
+    if pix_idx is not None:
+        pix_idx = pix_idx.to(device=pats.device, dtype=torch.int32).contiguous()
+        d = pix_idx.numel()
+        if d == 0 or int(pix_idx.min()) < 0 or int(pix_idx.max()) >= p:
+            raise ValueError("pixel indices must lie in [0, n_pixels)")
+    else:
+        d = p
+    out = alloc_patterns(n, d, DTYPE_F32, pats.device)
+    out2 = alloc_patterns(n, d, dtype2, pats.device)
+    flags = (FLAG_ZERO_MEAN if zero_mean else 0) | (FLAG_UNIT_NORM if unit_norm else 0)
+    st = _lib.load().ebsd_prep_patterns_pair(pats.data_ptr(), n, pats.stride(0), _lib.ptr(pix_idx), d,
+                                             out.data.data_ptr(), out.ld, out2.data.data_ptr(), dtype2, out2.ld,
+                                             flags, _lib.stream_ptr(pats.device))
+    check(st, "ebsd_prep_patterns_pair")
+    return out, out2
+
+
 def prep_patterns(pats: torch.Tensor, dtype: int = DTYPE_F32, zero_mean: bool = True, unit_norm: bool = False,
                   pix_idx: Optional[torch.Tensor] = None, out: Optional[PatternMatrix] = None) -> PatternMatrix:
     """Experimental-pattern preparation (ebsd_prep_patterns).

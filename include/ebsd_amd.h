@@ -136,6 +136,19 @@ int ebsd_prep_patterns(const float* pats, int n, int ld_in, const int* pix_idx,
                        int n_pix, void* out, int out_dtype, int ld_out,
                        float* out_scale, int flags, void* stream);
 
+/*
+ * ebsd_prep_patterns with two outputs from one read of the patterns: the
+ * finished rows in f32 (out_f32, ld_f32) and the same values converted to
+ * out2_dtype (EBSD_DTYPE_BF16 or EBSD_DTYPE_F16; out2, ld2), bit for bit what
+ * two ebsd_prep_patterns calls would write.
+ * Replaces: ebsd/indexing.py:175-187, as ebsd_prep_patterns; the second
+ * output has no counterpart in the reference (operands of the projection of
+ * queries onto the principal directions of a compressed search).
+ */
+int ebsd_prep_patterns_pair(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix,
+                            float* out_f32, int ld_f32, void* out2, int out2_dtype, int ld2, int flags,
+                            void* stream);
+
 /* Bytes of scratch that ebsd_sim_topk needs for these sizes. */
 int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t* bytes);
 

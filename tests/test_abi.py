@@ -51,6 +51,10 @@ def test_argument_errors_return_codes_without_launching():
                             None) == _lib.EBSD_ERR_ALIGN
     assert lib.ebsd_prep_patterns(None, 1, 4, None, 4, fake, 0, 32, None, 1, None) == _lib.EBSD_ERR_NULL
     assert lib.ebsd_prep_patterns(fake, 1, 4, None, 8, fake, 0, 32, None, 1, None) == _lib.EBSD_ERR_SHAPE
+    assert lib.ebsd_prep_patterns_pair(None, 1, 4, None, 4, fake, 32, fake, 1, 64, 1, None) == _lib.EBSD_ERR_NULL
+    assert lib.ebsd_prep_patterns_pair(fake, 1, 4, None, 4, fake, 32, fake, 3, 128, 1, None) == _lib.EBSD_ERR_DTYPE
+    assert lib.ebsd_prep_patterns_pair(fake, 1, 4, None, 4, fake, 32, fake, 1, 32, 1, None) == _lib.EBSD_ERR_SHAPE
+    assert lib.ebsd_prep_patterns_pair(fake, 1, 4, None, 8, fake, 32, fake, 1, 64, 1, None) == _lib.EBSD_ERR_SHAPE
     assert lib.ebsd_sample_dirs(fake, 5, 10, None, 3, fake, 0, None) == _lib.EBSD_ERR_NULL
     nb = ctypes.c_size_t(0)
     assert lib.ebsd_sim_topk_workspace(10, 100, 17, 0, ctypes.byref(nb)) == _lib.EBSD_ERR_TOPK

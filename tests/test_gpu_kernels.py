@@ -108,6 +108,23 @@ def test_prep_patterns_mean_mask_and_norm():
     assert torch.allclose(got, z / z.norm(dim=1, keepdim=True), atol=1e-6)
 
 
+@pytest.mark.parametrize("dtype2", [DTYPE_BF16, DTYPE_F16])
+@pytest.mark.parametrize("p,masked,unit_norm", [(3600, False, False), (3600, True, True), (100, False, True),
+                                                (14400, True, False)])
+def test_prep_patterns_pair_equals_two_preps(dtype2, p, masked, unit_norm):
+    """ebsd_prep_patterns_pair writes bit for bit what two ebsd_prep_patterns
+    calls (FP32 and dtype2) write, padding included."""
+    _say(f"prep pair dtype2={dtype2} p={p} masked={masked} unit_norm={unit_norm}")
+    g = torch.Generator().manual_seed(p + dtype2)
+    x = (torch.rand(45, p, generator=g) * 3 + 1).to(DEV)
+    idx = torch.nonzero(torch.rand(p, generator=g) > 0.3).flatten().to(DEV) if masked else None
+    f, h = kernels.prep_patterns_pair(x, dtype2, unit_norm=unit_norm, pix_idx=idx)
+    f_ref = kernels.prep_patterns(x, DTYPE_F32, unit_norm=unit_norm, pix_idx=idx)
+    h_ref = kernels.prep_patterns(x, dtype2, unit_norm=unit_norm, pix_idx=idx)
+    assert f.ld == f_ref.ld and h.ld == h_ref.ld
+    assert torch.equal(f.data, f_ref.data) and torch.equal(h.data.view(torch.int16), h_ref.data.view(torch.int16))
+
+
 def _check_topk(val, idx, q, d, k, tol=1e-4):
     """val/idx against float64 brute force, allowing swaps only between
     near-equal scores."""
