@@ -43,6 +43,51 @@ __device__ __forceinline__ float block_sum_256(float v, float* scratch) {
   return (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
 }
 
+// Sum / maximum over a workgroup of BLOCK threads (BLOCK / 64 slots of
+// scratch); BLOCK = 256 is block_sum_256 / block_max_256. The wave partials
+// are combined in a fixed pairwise order.
+template <int BLOCK>
+__device__ __forceinline__ float block_sum(float v, float* scratch) {
+  if constexpr (BLOCK == 256) {
+    return block_sum_256(v, scratch);
+  } else {
+    constexpr int W = BLOCK / 64;
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch[w] = v;
+    __syncthreads();
+    float p[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) p[i] = scratch[i];
+#pragma unroll
+    for (int n = W / 2; n > 0; n >>= 1)
+#pragma unroll
+      for (int i = 0; i < n; ++i) p[i] = p[2 * i] + p[2 * i + 1];
+    return p[0];
+  }
+}
+
+__device__ __forceinline__ float block_max_256(float v, float* scratch);
+
+template <int BLOCK>
+__device__ __forceinline__ float block_max(float v, float* scratch) {
+  if constexpr (BLOCK == 256) {
+    return block_max_256(v, scratch);
+  } else {
+    constexpr int W = BLOCK / 64;
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch[w] = v;
+    __syncthreads();
+    float m = scratch[0];
+#pragma unroll
+    for (int i = 1; i < W; ++i) m = fmaxf(m, scratch[i]);
+    return m;
+  }
+}
+
 __device__ __forceinline__ float block_max_256(float v, float* scratch) {
   v = wave_max(v);
   const int w = threadIdx.x >> 6;

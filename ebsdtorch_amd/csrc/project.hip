@@ -48,7 +48,7 @@ template <> struct is_fp8<uint8_t> { static constexpr bool value = true; };
 
 // Normalise the ITEMS register values of one pattern (n_valid of them are
 // real pixels) and store row `row` of `out`, zero-padding to ld.
-template <int ITEMS, typename OutT>
+template <int ITEMS, typename OutT, int BLOCK = 256>
 __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, int flags,
                                            OutT* __restrict__ out, int ld, float* __restrict__ out_scale,
                                            size_t row, float* red) {
@@ -57,10 +57,10 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
     float s = 0.0f;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) s += vals[i];  // padding slots hold 0
-    const float mean = block_sum_256(s, red) / (float)n_valid;
+    const float mean = block_sum<BLOCK>(s, red) / (float)n_valid;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const int d = tid + i * 256;
+      const int d = tid + i * BLOCK;
       vals[i] = (d < n_valid) ? (vals[i] - mean) : 0.0f;
     }
   }
@@ -68,7 +68,7 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
     float ss = 0.0f;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) ss += vals[i] * vals[i];
-    const float tot = block_sum_256(ss, red);
+    const float tot = block_sum<BLOCK>(ss, red);
     const float inv = tot > 0.0f ? 1.0f / sqrtf(tot) : 0.0f;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) vals[i] *= inv;
@@ -78,7 +78,7 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
     float am = 0.0f;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) am = fmaxf(am, fabsf(vals[i]));
-    am = block_max_256(am, red);
+    am = block_max<BLOCK>(am, red);
     // the smallest power of two 2^k with am / 2^k <= 448 (the row's largest
     // magnitude lands in (224, 448]): an E8M0 block scale for the FP8
     // matrix instructions, and an exact multiplier
@@ -96,11 +96,11 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
   OutT* orow = out + row * (size_t)ld;
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const int d = tid + i * 256;
+    const int d = tid + i * BLOCK;
     if (d < ld) orow[d] = to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f);
   }
-  // Columns beyond ITEMS*256 (only when ld exceeds it) are zero padding.
-  for (int d = ITEMS * 256 + tid; d < ld; d += 256) orow[d] = to_out<OutT>(0.0f);
+  // Columns beyond ITEMS*BLOCK (only when ld exceeds it) are zero padding.
+  for (int d = ITEMS * BLOCK + tid; d < ld; d += BLOCK) orow[d] = to_out<OutT>(0.0f);
 }
 
 // Persistent: workgroup g projects patterns g, g + gridDim.x, ... With at
@@ -171,50 +171,53 @@ __global__ __launch_bounds__(256) void project_kernel(
   }
 }
 
-template <int ITEMS, typename OutT>
-__global__ __launch_bounds__(256) void prep_kernel(
+// One workgroup of BLOCK threads per row, ITEMS values per thread (rows of
+// up to 4,096 pixels: 256 threads; longer rows: 1,024 threads, so that a
+// thread holds at most 16 values and occupancy stays up).
+template <int ITEMS, int BLOCK, typename OutT>
+__global__ __launch_bounds__(BLOCK) void prep_kernel(
     const float* __restrict__ pats, int ld_in, const int* __restrict__ pix_idx, int n_pix,
     OutT* __restrict__ out, int ld, float* __restrict__ out_scale, int flags) {
-  __shared__ float red[4];
+  __shared__ float red[BLOCK / 64];
   const size_t n = blockIdx.x;
   const float* row = pats + n * (size_t)ld_in;
   float vals[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const int d = threadIdx.x + i * 256;
+    const int d = threadIdx.x + i * BLOCK;
     float v = 0.0f;
-    if (d < n_pix) v = row[pix_idx ? pix_idx[d] : d];
+    if (d < n_pix) v = __builtin_nontemporal_load(&row[pix_idx ? pix_idx[d] : d]);  // read once
     vals[i] = v;
   }
-  finish_row<ITEMS, OutT>(vals, n_pix, flags, out, ld, out_scale, n, red);
+  finish_row<ITEMS, OutT, BLOCK>(vals, n_pix, flags, out, ld, out_scale, n, red);
 }
 
 // prep_kernel with a second output: the finished FP32 row and the same
 // values converted to Out2 (BF16 / FP16), from one read of the patterns.
-template <int ITEMS, typename Out2>
-__global__ __launch_bounds__(256) void prep_pair_kernel(
+template <int ITEMS, int BLOCK, typename Out2>
+__global__ __launch_bounds__(BLOCK) void prep_pair_kernel(
     const float* __restrict__ pats, int ld_in, const int* __restrict__ pix_idx, int n_pix,
     float* __restrict__ out, int ld, Out2* __restrict__ out2, int ld2, int flags) {
-  __shared__ float red[4];
+  __shared__ float red[BLOCK / 64];
   const size_t n = blockIdx.x;
   const float* row = pats + n * (size_t)ld_in;
   float vals[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const int d = threadIdx.x + i * 256;
+    const int d = threadIdx.x + i * BLOCK;
     float v = 0.0f;
-    if (d < n_pix) v = row[pix_idx ? pix_idx[d] : d];
+    if (d < n_pix) v = __builtin_nontemporal_load(&row[pix_idx ? pix_idx[d] : d]);  // read once
     vals[i] = v;
   }
   // finish_row leaves the finished values in vals (FP32 output: no scale)
-  finish_row<ITEMS, float>(vals, n_pix, flags, out, ld, nullptr, n, red);
+  finish_row<ITEMS, float, BLOCK>(vals, n_pix, flags, out, ld, nullptr, n, red);
   Out2* orow = out2 + n * (size_t)ld2;
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const int d = threadIdx.x + i * 256;
+    const int d = threadIdx.x + i * BLOCK;
     if (d < ld2) orow[d] = to_out<Out2>(d < n_pix ? vals[i] : 0.0f);
   }
-  for (int d = ITEMS * 256 + threadIdx.x; d < ld2; d += 256) orow[d] = to_out<Out2>(0.0f);
+  for (int d = ITEMS * BLOCK + threadIdx.x; d < ld2; d += BLOCK) orow[d] = to_out<Out2>(0.0f);
 }
 
 // Sample the master pattern at arbitrary sphere directions (one per thread).
@@ -257,21 +260,19 @@ int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, 
 template <typename Out2>
 int launch_prep_pair(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix, float* out, int ld,
                      void* out2, int ld2, int flags, hipStream_t st) {
-  const dim3 grid(n), block(256);
   Out2* o2 = static_cast<Out2*>(out2);
-  const int items = (n_pix + 255) / 256;
-#define EBSD_PREP_PAIR_CASE(I)                                                                          \
-  if (items <= I) {                                                                                     \
-    hipLaunchKernelGGL((prep_pair_kernel<I, Out2>), grid, block, 0, st, pats, ld_in, pix_idx, n_pix, out, ld, \
-                       o2, ld2, flags);                                                                 \
-    EBSD_LAUNCH_CHECK();                                                                                \
-    return EBSD_OK;                                                                                     \
+#define EBSD_PREP_PAIR_CASE(I, B)                                                                         \
+  if (n_pix <= (I) * (B)) {                                                                               \
+    hipLaunchKernelGGL((prep_pair_kernel<I, B, Out2>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx, n_pix, \
+                       out, ld, o2, ld2, flags);                                                          \
+    EBSD_LAUNCH_CHECK();                                                                                  \
+    return EBSD_OK;                                                                                       \
   }
-  EBSD_PREP_PAIR_CASE(4)
-  EBSD_PREP_PAIR_CASE(8)
-  EBSD_PREP_PAIR_CASE(16)
-  EBSD_PREP_PAIR_CASE(32)
-  EBSD_PREP_PAIR_CASE(64)
+  EBSD_PREP_PAIR_CASE(4, 256)
+  EBSD_PREP_PAIR_CASE(8, 256)
+  EBSD_PREP_PAIR_CASE(16, 256)
+  EBSD_PREP_PAIR_CASE(8, 1024)
+  EBSD_PREP_PAIR_CASE(16, 1024)
 #undef EBSD_PREP_PAIR_CASE
   return EBSD_ERR_SHAPE;
 }
@@ -279,21 +280,19 @@ int launch_prep_pair(const float* pats, int n, int ld_in, const int* pix_idx, in
 template <typename OutT>
 int launch_prep(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix, void* out,
                 int ld, float* out_scale, int flags, hipStream_t st) {
-  const dim3 grid(n), block(256);
   OutT* o = static_cast<OutT*>(out);
-  const int items = (n_pix + 255) / 256;
-#define EBSD_PREP_CASE(I)                                                                      \
-  if (items <= I) {                                                                            \
-    hipLaunchKernelGGL((prep_kernel<I, OutT>), grid, block, 0, st, pats, ld_in, pix_idx, n_pix, \
-                       o, ld, out_scale, flags);                                               \
-    EBSD_LAUNCH_CHECK();                                                                       \
-    return EBSD_OK;                                                                            \
+#define EBSD_PREP_CASE(I, B)                                                                          \
+  if (n_pix <= (I) * (B)) {                                                                           \
+    hipLaunchKernelGGL((prep_kernel<I, B, OutT>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx, n_pix, \
+                       o, ld, out_scale, flags);                                                      \
+    EBSD_LAUNCH_CHECK();                                                                              \
+    return EBSD_OK;                                                                                   \
   }
-  EBSD_PREP_CASE(4)
-  EBSD_PREP_CASE(8)
-  EBSD_PREP_CASE(16)
-  EBSD_PREP_CASE(32)
-  EBSD_PREP_CASE(64)
+  EBSD_PREP_CASE(4, 256)
+  EBSD_PREP_CASE(8, 256)
+  EBSD_PREP_CASE(16, 256)
+  EBSD_PREP_CASE(8, 1024)
+  EBSD_PREP_CASE(16, 1024)
 #undef EBSD_PREP_CASE
   return EBSD_ERR_SHAPE;
 }

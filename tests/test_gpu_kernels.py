@@ -92,13 +92,16 @@ def test_projection_low_precision_outputs(mp101, dtype):
         assert torch.equal(lp.data[:, :3600], f32.to(tdt))
 
 
-def test_prep_patterns_mean_mask_and_norm():
-    _say("prep patterns")
-    g = torch.Generator().manual_seed(4)
-    x = torch.rand(37, 3600, generator=g) * 3 + 1
+@pytest.mark.parametrize("p", [3600, 14400, 5000])
+def test_prep_patterns_mean_mask_and_norm(p):
+    """Rows of up to 4,096 pixels take 256-thread workgroups, longer rows
+    1,024 threads (8 or 16 values per thread)."""
+    _say(f"prep patterns p={p}")
+    g = torch.Generator().manual_seed(4 + p)
+    x = torch.rand(37, p, generator=g) * 3 + 1
     got = kernels.prep_patterns(x.to(DEV)).to_float().cpu()
     assert torch.allclose(got, x - x.mean(1, keepdim=True), atol=2e-6)
-    idx = torch.nonzero(torch.rand(3600, generator=g) > 0.3).flatten()
+    idx = torch.nonzero(torch.rand(p, generator=g) > 0.3).flatten()
     got = kernels.prep_patterns(x.to(DEV), pix_idx=idx.to(DEV)).to_float().cpu()
     xs = x[:, idx]
     assert got.shape[1] == len(idx)
@@ -106,6 +109,11 @@ def test_prep_patterns_mean_mask_and_norm():
     got = kernels.prep_patterns(x.to(DEV), unit_norm=True).to_float().cpu().double()
     z = x.double() - x.double().mean(1, keepdim=True)
     assert torch.allclose(got, z / z.norm(dim=1, keepdim=True), atol=1e-6)
+    # FP8 rows: power-of-two scale, the largest magnitude in (224, 448]
+    q8 = kernels.prep_patterns(x.to(DEV), DTYPE_FP8_E4M3)
+    amax = q8.data[:, :p].view(torch.float8_e4m3fn).float().abs().amax(1)
+    assert torch.all(amax > 224) and torch.all(amax <= 448)
+    assert torch.allclose(q8.to_float().cpu().double(), z, rtol=0.07, atol=0.07 * z.abs().max().item() / 16)
 
 
 @pytest.mark.parametrize("dtype2", [DTYPE_BF16, DTYPE_F16])
