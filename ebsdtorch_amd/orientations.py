@@ -131,11 +131,19 @@ def qu_conj(q: torch.Tensor) -> torch.Tensor:
 
 def qu_apply(q: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
     """Rotate points v (..., 3) by unit quaternions q (..., 4) (broadcast):
-    v' = v + 2w (u x v) + 2 u x (u x v)."""
-    w = q[..., :1]
-    u, v = torch.broadcast_tensors(q[..., 1:], v)
-    t = 2.0 * torch.cross(u, v, dim=-1)
-    return v + w * t + torch.cross(u, t, dim=-1)
+    the vector part of q (1, v) q*, expanded term by term in the rounding
+    order of quaternions.py:186-213 (the projection kernel uses the same
+    order), so float32 results equal the reference's bit for bit. The 1 in
+    the scalar slot only adds |q|^2 to the scalar part, which is dropped."""
+    aw, ax, ay, az = q.unbind(-1)
+    bx, by, bz = v.unbind(-1)
+    pw = aw - ax * bx - ay * by - az * bz
+    px = aw * bx + ax + ay * bz - az * by
+    py = aw * by - ax * bz + ay + az * bx
+    pz = aw * bz + ax * by - ay * bx + az
+    return torch.stack((-pw * ax + px * aw - py * az + pz * ay,
+                        -pw * ay + px * az + py * aw - pz * ax,
+                        -pw * az - px * ay + py * ax + pz * aw), -1)
 
 
 # --------------------------------------------------------------------------
