@@ -114,10 +114,10 @@ class DictionaryIndexer(_ResidentSearch):
         exactly against the full FP32 dictionary. The second moment is
         accumulated on the matrix cores (ebsd_cov_update: centred aggregate,
         plus n mean mean^T) over up to `sample` dictionary rows and solved in
-        FP64 (on the host up to D = 4096). components "auto": the smallest multiple of 64 (at least the
+        FP64 on the host (fp64_eigh). components "auto": the smallest multiple of 64 (at least the
         search kernel's 4 K stages) whose leading eigenvalues hold
         `explained` of the trace; an int: that many."""
-        from .pca_covmat import OnlineCovMatrix
+        from .pca_covmat import OnlineCovMatrix, fp64_eigh
 
         full = self.dictionary_f32
         d = full.n_pix
@@ -128,10 +128,9 @@ class DictionaryIndexer(_ResidentSearch):
         n_obs = float(int(cov.obs.item()))
         mean = cov.mean.to(torch.float64)
         m2 = cov._full_aggregate().to(torch.float64) + n_obs * torch.outer(mean, mean)
-        # one FP64 solve per dictionary: on the host (LAPACK, ~1-2 s) up to
-        # D = 4096, on the device above (the host solve grows as D^3)
-        evals, evecs = torch.linalg.eigh(m2.cpu() if d <= 4096 else m2)
-        evals, evecs = evals.clamp_min(0.0).to(self.device), evecs.to(self.device)
+        # one FP64 solve per dictionary, on the host (fp64_eigh)
+        evals, evecs = fp64_eigh(m2)
+        evals = evals.clamp_min(0.0)
         elem = kernels.TORCH_DTYPE[self.dtype].itemsize
         k_min = max(64, -(-256 // elem))  # >= 4 K stages of 64 bytes
         if components == "auto":

@@ -4,8 +4,8 @@ Same interface as ebsdtorch's OnlineCovMatrix: call the object on batches
 (B, n_features), then get_covmat() / get_eigenvectors(). Batches are merged
 by the HIP kernel ebsd_cov_update (Chan's batched Welford update, FP32
 accumulation on the matrix cores); eigenvectors come from an FP64 symmetric
-eigensolver (torch.linalg.eigh on the device), ascending eigenvalues as in
-the reference.
+eigensolver on the host (fp64_eigh), ascending eigenvalues as in the
+reference.
 """
 
 from __future__ import annotations
@@ -14,6 +14,23 @@ import torch
 
 from . import kernels
 from ._lib import DTYPE_F32
+
+
+def fp64_eigh(matrix: torch.Tensor, dtype: torch.dtype = torch.float64):
+    """(eigenvalues ascending, eigenvectors as columns) of a symmetric
+    matrix, solved in float64 (or `dtype`) by the host's LAPACK
+    (torch.linalg.eigh on the CPU) and returned on the matrix's device.
+
+    Every eigendecomposition of this package goes through here. The device
+    solver (hipSOLVER syevd through torch) is not used: one profiled run
+    (rocprofv3 with hardware counters) ended with a segmentation fault inside
+    hipsolverDsyevd on a 3,600 x 3,600 matrix, with no cause established, so
+    the solve stays on the host at every size. It runs once per dictionary
+    (or covariance) and costs about 9 D^3 flops: ~1 s at D = 3,600, ~8 s at
+    D = 6,400 and about a minute at D = 14,400 on 8-16 host cores."""
+    dev = matrix.device
+    evals, evecs = torch.linalg.eigh(matrix.detach().to("cpu", dtype))
+    return evals.to(dev), evecs.to(dev)
 
 
 class OnlineCovMatrix(torch.nn.Module):
@@ -61,8 +78,10 @@ class OnlineCovMatrix(torch.nn.Module):
         return cov
 
     def get_eigenvectors(self, dtype: torch.dtype = torch.float64) -> torch.Tensor:
-        """Eigenvectors as columns, ascending eigenvalues; solved in FP64."""
-        return torch.linalg.eigh(self.get_covmat().to(dtype))[1]
+        """Eigenvectors as columns, ascending eigenvalues (pca_covmat.py:149-158);
+        solved in FP64 on the host (fp64_eigh)."""
+        return fp64_eigh(self.get_covmat(), dtype)[1]
 
     def get_eigensystem(self, dtype: torch.dtype = torch.float64):
-        return torch.linalg.eigh(self.get_covmat().to(dtype))
+        """(eigenvalues ascending, eigenvectors as columns), FP64 host solve."""
+        return fp64_eigh(self.get_covmat(), dtype)
