@@ -14,10 +14,16 @@ knn.py:80-287, and PCA DI, indexing.py:214-476) on the same seeded inputs.
   and FP8 on principal coordinates + FP32 rescore against FP32.
 
 The tie rule: two different candidates at one rank are accepted only when
-their scores differ by less than 1e-6 in cosine units, i.e.
-|s_a - s_b| / (|q| max(|d_a|, |d_b|)) < 1e-6 for the zero-mean query q and
-dictionary patterns d (FP64, from the oracle's projection). For cosine
-ranking (unit-norm dictionary) this is the cosine difference itself.
+their scores differ by less than a tie threshold in cosine units, i.e.
+|s_a - s_b| / (|q| max(|d_a|, |d_b|)) for the zero-mean query q and
+dictionary patterns d (FP64, from the oracle's projection); for cosine
+ranking (unit-norm dictionary) this is the cosine difference itself. At
+rank 1 (the top-1 index, the parity bar) the threshold is 1e-6. Below it
+the candidates are dense, and both sides score in FP32: the GPU's and the
+oracle's FP32 scores of the same entry differ by up to ~1e-6 in cosine
+units at these sizes (e.g. 210.1651 vs 210.1654 at |q||d| ~ 350), so two
+FP32 rankings can order a pair whose exact scores are 1.3e-6 apart either
+way; ranks 2-16 are compared with a threshold of 4e-6.
 """
 
 import time
@@ -32,6 +38,7 @@ from oracle import di_oracle as O
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda")
 TIE = 1e-6
+RANK_TIE = 4e-6
 
 
 def _say(*a):
@@ -69,12 +76,15 @@ def _check_ranks(got_idx, got_val, ref_idx, ref_val, queries, dict_rows, what):
         rows = dict_rows(need)
         pos = {int(v): i for i, v in enumerate(need.tolist())}
         worst = 0.0
+        n_top1 = 0
         for n, r in diff.tolist():
             q = _zm(queries[n])
             gap = _tie_gap(q, rows[pos[int(got_idx[n, r])]], rows[pos[int(ref_idx[n, r])]])
             worst = max(worst, gap)
-            assert gap < TIE, f"{what}: pattern {n} rank {r}: {int(got_idx[n, r])} vs {int(ref_idx[n, r])}, gap {gap:.3e}"
-        _say(f"{what}: {len(diff)} rank differences, all ties (worst {worst:.2e})")
+            n_top1 += r == 0
+            assert gap < (TIE if r == 0 else RANK_TIE), \
+                f"{what}: pattern {n} rank {r + 1}: {int(got_idx[n, r])} vs {int(ref_idx[n, r])}, gap {gap:.3e}"
+        _say(f"{what}: {len(diff)} rank differences ({n_top1} at rank 1), all ties (worst {worst:.2e})")
     else:
         _say(f"{what}: identical top-{got_idx.shape[1]} lists")
     scale = ref_val.abs().amax(1, keepdim=True)
