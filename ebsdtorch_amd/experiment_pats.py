@@ -135,8 +135,10 @@ class ExperimentPatterns(torch.nn.Module):
         comparison of (N, k) against (N,) metrics does not broadcast)."""
         if not hasattr(self, "raw_indexing_results"):
             raise ValueError("Raw indexing results must be set before combining.")
-        dev = self.patterns.device
         res = self.raw_indexing_results
+        # results live where the indexing ran (the GPU), which need not be
+        # where the patterns are kept: combine on the first phase's device
+        dev = next(iter(res.values()))[1].device
         if len(res) == 1:
             pid = next(iter(res))
             self.phases = torch.full((self.n_patterns,), pid, dtype=torch.uint8, device=dev)
@@ -147,6 +149,7 @@ class ExperimentPatterns(torch.nn.Module):
         phases = torch.full((self.n_patterns,), 255, dtype=torch.uint8, device=dev)
         ori = torch.zeros(self.n_patterns, 4, device=dev)
         for pid, (q, m) in res.items():
+            q, m = q.to(dev), m.to(dev)
             m1 = m[:, 0].to(best.dtype)
             mask = m1 > best if higher_is_better else m1 < best
             best[mask] = m1[mask]

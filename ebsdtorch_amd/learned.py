@@ -473,6 +473,10 @@ class LearnedIndexer:
         self.quads = kernels.mp_quad_table(self.mp, quad_dtype)
         if fallback is not None and fallback.n_pix != self.rays.shape[0]:
             raise ValueError("the fallback dictionary must use the same detector pixels")
+        if fallback is not None and not getattr(fallback, "unit_norm", False):
+            # the better of the local cosine and the fallback's score is kept,
+            # which compares like with like only when both are cosines
+            raise ValueError("the fallback DictionaryIndexer must be built with dict_unit_norm=True")
 
     def _local_queries(self, x: torch.Tensor) -> kernels.PatternMatrix:
         if self.binning == 1:

@@ -166,3 +166,40 @@ def test_pca_compressed_search_keeps_fp32_top1(components):
     # the scores are the exact FP32 ones of the reported entries
     torch.testing.assert_close(val[:, 0], (qf.to_float() * ref.dictionary.to_float()[idx[:, 0].long()]).sum(-1),
                                atol=1e-4, rtol=1e-5)
+
+
+def test_two_phase_driver_with_cpu_resident_patterns():
+    """Two phases (an m-3m and a 6/mmm master pattern), patterns kept on the
+    host as the reference usually holds them: per phase the top-1 equals the
+    oracle's (ties aside), and the combined result takes, per pattern, the
+    phase whose top-1 metric is larger (experiment_pats.py:387-451)."""
+    _say("two-phase driver")
+    mps = [synthetic.band_master_pattern(101, 11, 30, 2.0, seed=3),
+           synthetic.band_master_pattern(101, 9, 30, 2.0, seed=4)]
+    dicts = [O.sample_ori_fz_laue(11, 3000), O.sample_ori_fz_laue(9, 3000)]
+    geom = E.EBSDGeometry((30, 30), proj_center=(0.5, 0.5, 0.6))
+    rays = O.normalized_rays((30, 30), (0.5, 0.5, 0.6))
+    ori = synthetic.random_orientations(96, seed=8)
+    pats = torch.cat([O.interpolate(mps[0], O.qu_apply(ori[:48, None], rays[None]), normalize_coords=False),
+                      O.interpolate(mps[1], O.qu_apply(ori[48:, None], rays[None]), normalize_coords=False)])
+    pats = pats + 0.01 * torch.randn(pats.shape, generator=torch.Generator().manual_seed(9))
+    exp = E.ExperimentPatterns(pats.view(96, 30, 30))  # host-resident
+    E.dictionary_index_orientations([E.MasterPattern(m.to(DEV), laue_group=l) for m, l in zip(mps, (11, 9))],
+                                    geom, exp, top_k_matches=2, match_dtype=torch.float32,
+                                    dictionary_orientations=dicts)
+    ref = [O.dictionary_index(m, rays, pats, d, top_k=1, dictionary_chunk_size=1000, experiment_chunk_size=1000)
+           for m, d in zip(mps, dicts)]
+    for p in range(2):
+        dict_pats = O.project_dictionary(mps[p], rays, dicts[p], chunk=1000)
+        _assert_top1_parity(exp.raw_indexing_indices[p][:, 0].cpu(), ref[p][0][:, 0], pats, dict_pats)
+    m0 = exp.raw_indexing_results[0][1][:, 0].cpu()
+    m1 = exp.raw_indexing_results[1][1][:, 0].cpu()
+    want = (m1 > m0).to(torch.uint8)
+    assert torch.equal(exp.phases.cpu(), want)
+    # each pattern was simulated from one phase; it must be recognised
+    assert torch.equal(want, (torch.arange(96) >= 48).to(torch.uint8))
+    best = torch.where(want.bool(), m1, m0)
+    assert torch.equal(exp.best_metrics.cpu(), best)
+    q_sel = torch.where(want.bool()[:, None], exp.raw_indexing_results[1][0][:, 0].cpu(),
+                        exp.raw_indexing_results[0][0][:, 0].cpu())
+    assert torch.equal(exp.orientations.cpu(), q_sel)

@@ -71,3 +71,47 @@ def test_trained_network_gain(world):
     assert res["trained"][0] < 10.0 < res["untrained"][0]
     assert res["trained"][1] > res["untrained"][1] + 0.3
     assert res["trained"][2] < res["untrained"][2]
+
+
+def test_indexer_with_full_dictionary_fallback(world):
+    """LearnedIndexer with its fallback: a pattern whose local cosine is
+    below the threshold (or whose best local orientation is on the ball's
+    shell) is searched against the full dictionary (FP32, cosine ranking),
+    and the better of the two cosines is kept. An untrained network sends
+    most patterns to the fallback, so the result must then be plain FP32 DI's
+    orientation wherever DI's cosine is the better one. A fallback that does
+    not rank by cosine is refused."""
+    import ebsdtorch_amd as E
+    from ebsdtorch_amd.learned import LearnedIndexer, ProposalNet
+    from ebsdtorch_amd.orientations import disori_angle_laue
+
+    mp, geo = world
+    rays = geo.get_rays().to(DEV)
+    dico = E.sample_ori_fz_laue(11, 50_000, DEV, permute=True, generator=torch.Generator().manual_seed(3))
+    net = ProposalNet(SHAPE, DEV, laue_id=11, compute="bf16", seed=4)
+    with pytest.raises(ValueError):
+        LearnedIndexer(net, mp, rays, 11, fallback=E.DictionaryIndexer(mp, rays, dico), threshold=0.5)
+    fb = E.DictionaryIndexer(mp, rays, dico, precision="fp32", dict_unit_norm=True)
+    idx = LearnedIndexer(net, mp, rays, 11, fallback=fb, threshold=0.5)
+    x, q_true = _sim(world, 77).batch(2048)
+    res = idx.index(x)
+    flagged = res["fallback"]
+    print(f"fallback rate {float(flagged.float().mean()):.3f}", flush=True)
+    assert float(flagged.float().mean()) > 0.5
+    # plain FP32 DI of the same dictionary, cosine ranking
+    qv, qi = fb.index(fb.prepare_pair(x, unit_norm=True)[0], 1)
+    di_q = dico[qi[:, 0].long()]
+    di_cos = qv[:, 0]
+    loc = res["local_scores"]
+    took_di = flagged & (di_cos > loc)
+    assert torch.all(res["scores"][took_di] == di_cos[took_di])
+    d = disori_angle_laue(res["orientations"][took_di].double(), di_q[took_di].double(), 11)
+    assert float(d.max()) < 2e-3  # acos of 1 - O(1e-7): float32-rounded symmetry table
+    # elsewhere the local result is kept, and the kept score is never below
+    # either candidate's
+    assert torch.all(res["scores"][~took_di] == loc[~took_di])
+    assert torch.all(res["scores"][flagged] >= torch.maximum(loc, di_cos)[flagged] - 1e-6)
+    err = torch.rad2deg(disori_angle_laue(res["orientations"].double(), q_true.double(), 11))
+    err_di = torch.rad2deg(disori_angle_laue(di_q.double(), q_true.double(), 11))
+    print(f"median error {float(err.median()):.2f} deg (FP32 DI alone {float(err_di.median()):.2f} deg)")
+    assert abs(float(err[took_di].median()) - float(err_di[took_di].median())) < 0.1

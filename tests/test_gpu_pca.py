@@ -80,8 +80,9 @@ def test_pca_driver_matches_reference_golden(golden):
 
 def test_pca_indexing_against_fp64_oracle():
     """PCA DI (1600-pixel detector, 300 components) against the oracle's
-    PCA DI with an FP64 covariance and eigensolver: top-1 equal except near
-    ties (1e-5 relative score gap), and at least 99% equal overall."""
+    PCA DI with an FP64 covariance and eigensolver: top-1 equal except ties
+    within 1e-6 in cosine units, measured in the oracle's FP64 PCA space
+    (|s_a - s_b| / (|P q| max(|P d_a|, |P d_b|)))."""
     _say("PCA vs FP64 oracle")
     mp = synthetic.band_master_pattern(151, 11, 40, 2.0, seed=0)
     coords = O.detector_coords((40, 40), proj_center=(0.5, 0.5, 0.6))
@@ -97,8 +98,19 @@ def test_pca_indexing_against_fp64_oracle():
     eng = E.PCADictionaryIndexer(mp.to(DEV), coords.to(DEV), learn.to(DEV), 300)
     val, idx = eng.index(eng.prepare(exp.to(DEV)), 2)
     got = idx[:, 0].long().cpu()
-    agree = (got == ref_idx[:, 0]).float().mean().item()
-    _say(f"PCA top-1 agreement with the FP64 oracle: {agree:.4f}")
-    assert agree >= 0.99
-    gap = (ref_val[:, 0] - ref_val[:, 1]).abs() / ref_val[:, 0].abs()
-    assert torch.all((got == ref_idx[:, 0]) | (gap < 1e-3))
+    diff = torch.nonzero(got != ref_idx[:, 0]).flatten().tolist()
+    if diff:
+        cov = O.OnlineCovMatrix(coords.shape[0], covmat_dtype=torch.float64, delta_dtype=torch.float64)
+        for ob in torch.split(learn, 4096):
+            p = O.interpolate(mp, O.qu_apply(ob[:, None, :], coords[None]), normalize_coords=True).view(len(ob), -1)
+            cov.update((p - p.mean(-1, keepdim=True)).double())
+        basis = O.pca_components(cov.get_eigenvectors(), 300)
+        for n in diff:
+            cand = learn[torch.tensor([int(got[n]), int(ref_idx[n, 0])])]
+            p = O.interpolate(mp, O.qu_apply(cand[:, None, :], coords[None]), normalize_coords=True).view(2, -1)
+            pc = (p - p.mean(-1, keepdim=True)).double() @ basis
+            qc = (exp[n].double() - exp[n].double().mean()) @ basis
+            gap = float(abs(qc @ pc[0] - qc @ pc[1]) / (qc.norm() * pc.norm(dim=-1).max()))
+            assert gap < 1e-6, (n, int(got[n]), int(ref_idx[n, 0]), gap)
+    _say(f"PCA top-1: {len(diff)} of 500 differ from the FP64 oracle, all ties")
+    torch.testing.assert_close(val[:, 0].cpu(), ref_val[:, 0], rtol=1e-5, atol=1e-5)
