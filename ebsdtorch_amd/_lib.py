@@ -24,6 +24,8 @@ EBSD_ERR_DTYPE = -3
 EBSD_ERR_TOPK = -4
 EBSD_ERR_WORKSPACE = -5
 EBSD_ERR_ALIGN = -6
+EBSD_ERR_SCRATCH = -7
+EBSD_NOALLOC_SCRATCH_BYTES = 64 << 20
 EBSD_MAX_TOPK = 16
 
 DTYPE_F32 = 0
@@ -140,6 +142,11 @@ _SIGNATURES = {
         [_c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
          _c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_size_t, _c_void_p],
     ),
+    "ebsd_sim_topk_noalloc": (
+        _c_int,
+        [_c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
+         _c_void_p, _c_void_p, _c_int, _c_int, _c_void_p],
+    ),
 }
 
 _lock = threading.Lock()
@@ -184,7 +191,8 @@ def check(status: int, what: str) -> None:
         return
     lib = load()
     msg = f"{what}: {lib.ebsd_status_string(status).decode()} (status {status})"
-    if status in (EBSD_ERR_SHAPE, EBSD_ERR_TOPK, EBSD_ERR_NULL, EBSD_ERR_ALIGN, EBSD_ERR_WORKSPACE):
+    if status in (EBSD_ERR_SHAPE, EBSD_ERR_TOPK, EBSD_ERR_NULL, EBSD_ERR_ALIGN, EBSD_ERR_WORKSPACE,
+                  EBSD_ERR_SCRATCH):
         raise ValueError(msg)
     if status == EBSD_ERR_DTYPE:
         raise NotImplementedError(msg)

@@ -13,6 +13,7 @@ extern "C" const char* ebsd_status_string(int status) {
     case EBSD_ERR_TOPK: return "top-k outside [1, 16]";
     case EBSD_ERR_WORKSPACE: return "workspace missing or too small";
     case EBSD_ERR_ALIGN: return "pointer not 16-byte aligned";
+    case EBSD_ERR_SCRATCH: return "sizes exceed the static scratch buffer of the allocation-free call";
     default: break;
   }
   if (status > 0) return hipGetErrorString(static_cast<hipError_t>(status));

@@ -34,6 +34,7 @@
 // state).
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "mfma_tile.h"
@@ -988,4 +989,63 @@ extern "C" int ebsd_sim_topk(const void* q, const float* q_scale, int m, const v
                                              top_idx, index_base, merge, pval, pidx, st);
   }
   return EBSD_ERR_DTYPE;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the partial top-k lists live in a static device
+// buffer instead of a caller workspace. Every GPU that loads this library
+// holds its own copy of `scratch` (EBSD_NOALLOC_SCRATCH_BYTES) until the
+// process ends. Host code never uses the symbol's name as an address: the
+// per-device address comes from hipGetSymbolAddress, taken once per device
+// (with that device current) on the first call there. Nothing in the buffer
+// needs clearing: the similarity kernels write every partial list before the
+// merge kernel reads it. Calls that overlap on one GPU share the buffer and
+// race; callers serialise them (one stream, or events).
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) unsigned char scratch[EBSD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's copy of `scratch` (set up on first use).
+int scratch_address(void** out) {
+  int dev = 0;
+  EBSD_HIP_RET(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return EBSD_ERR_SHAPE;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    EBSD_HIP_RET(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return EBSD_ERR_NULL;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return EBSD_OK;
+}
+
+}  // namespace
+
+extern "C" int ebsd_sim_topk_noalloc(const void* q, const float* q_scale, int m, const void* dict,
+                                     const float* d_scale, int n, int ld, int dtype, int topk,
+                                     float* top_val, int* top_idx, int index_base, int merge, void* stream) {
+  // same argument checks as ebsd_sim_topk, then the size check against the
+  // static buffer before any device work
+  if (!q || !dict || !top_val || !top_idx) return EBSD_ERR_NULL;
+  if (m < 0 || n < 1 || ld < 1) return EBSD_ERR_SHAPE;
+  if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
+  const int eb = elem_bytes(dtype);
+  if (eb == 0) return EBSD_ERR_DTYPE;
+  if (ld % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
+  if (m == 0) return EBSD_OK;
+  const size_t need = workspace_bytes(plan_sim(m, n, topk, dtype, ld * eb), m);
+  if (need > (size_t)EBSD_NOALLOC_SCRATCH_BYTES) return EBSD_ERR_SCRATCH;
+  void* ptr = nullptr;
+  const int st = scratch_address(&ptr);
+  if (st != EBSD_OK) return st;
+  return ebsd_sim_topk(q, q_scale, m, dict, d_scale, n, ld, dtype, topk, top_val, top_idx, index_base, merge,
+                       ptr, (size_t)EBSD_NOALLOC_SCRATCH_BYTES, stream);
 }

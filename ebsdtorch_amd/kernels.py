@@ -234,6 +234,29 @@ def sim_topk(queries: PatternMatrix, dictionary: PatternMatrix, topk: int, top_v
     check(st, "ebsd_sim_topk")
 
 
+def sim_topk_noalloc(queries: PatternMatrix, dictionary: PatternMatrix, topk: int, top_val: torch.Tensor,
+                     top_idx: torch.Tensor, index_base: int = 0, merge: bool = True) -> None:
+    """sim_topk through the allocation-free entry point
+    (ebsd_sim_topk_noalloc): the partial lists use the library's static
+    device buffer, no workspace is allocated. Sizes beyond that buffer raise
+    ValueError; calls overlapping on one GPU must be serialised."""
+    if queries.dtype != dictionary.dtype or queries.ld != dictionary.ld:
+        raise ValueError("queries and dictionary must share a dtype and a leading dimension")
+    m, n = queries.rows, dictionary.rows
+    if top_val.shape != (m, topk) or top_idx.shape != (m, topk):
+        raise ValueError("top-k state must be (n_queries, topk)")
+    if top_val.dtype != torch.float32 or top_idx.dtype != torch.int32:
+        raise ValueError("top-k state must be float32 values and int32 indices")
+    if not (top_val.is_contiguous() and top_idx.is_contiguous()):
+        raise ValueError("top-k state must be contiguous")
+    lib = _lib.load()
+    st = lib.ebsd_sim_topk_noalloc(queries.data.data_ptr(), _lib.ptr(queries.scale), m, dictionary.data.data_ptr(),
+                                   _lib.ptr(dictionary.scale), n, queries.ld, queries.dtype, topk,
+                                   top_val.data_ptr(), top_idx.data_ptr(), index_base, 1 if merge else 0,
+                                   _lib.stream_ptr(queries.data.device))
+    check(st, "ebsd_sim_topk_noalloc")
+
+
 def rescore(queries_f32: PatternMatrix, dictionary_f32: PatternMatrix, candidates: torch.Tensor, topk: int):
     """Exact FP32 re-scoring of candidate lists (ebsd_rescore): returns
     (values (N, topk) f32, indices (N, topk) int32), best first."""

@@ -36,6 +36,7 @@ extern "C" {
 #define EBSD_ERR_TOPK (-4)        /* top-k outside [1, EBSD_MAX_TOPK]       */
 #define EBSD_ERR_WORKSPACE (-5)   /* workspace missing or too small         */
 #define EBSD_ERR_ALIGN (-6)       /* pointer not 16-byte aligned            */
+#define EBSD_ERR_SCRATCH (-7)     /* the sizes need more than the static scratch of an allocation-free call */
 
 #define EBSD_MAX_TOPK 16
 
@@ -172,6 +173,23 @@ int ebsd_sim_topk(const void* q, const float* q_scale, int m, const void* dict,
                   const float* d_scale, int n, int ld, int dtype, int topk,
                   float* top_val, int* top_idx, int index_base, int merge,
                   void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Allocation-free form of ebsd_sim_topk: same arguments and results, with the
+ * partial top-k lists in a static device buffer of EBSD_NOALLOC_SCRATCH_BYTES
+ * per GPU (held from the library's load until the process ends) instead of a
+ * caller workspace, so the call makes no allocation: for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). Sizes whose partial lists exceed the buffer (FP32 with
+ * top-k > 1: m > 65,536 queries; BF16 / FP16 / FP8 on the 192 x 256 kernel:
+ * m > 262,144) return EBSD_ERR_SCRATCH before any device work. Calls that
+ * overlap on one GPU share the buffer and race: serialise them.
+ * No counterpart in the reference.
+ */
+#define EBSD_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ebsd_sim_topk_noalloc(const void* q, const float* q_scale, int m, const void* dict,
+                          const float* d_scale, int n, int ld, int dtype, int topk,
+                          float* top_val, int* top_idx, int index_base, int merge, void* stream);
 
 /*
  * FP32 re-scoring of low-precision candidates: for each query row i, the
