@@ -15,12 +15,13 @@ against it.
 
 from __future__ import annotations
 
+import time
 from typing import List, Optional, Sequence, Union
 
 import torch
 
 from . import kernels
-from ._lib import DTYPE_BF16, DTYPE_F32, DTYPE_FP8_E4M3
+from ._lib import DTYPE_BF16, DTYPE_F32, DTYPE_FP8_E4M3, EBSD_MAX_TOPK
 from .experiment_pats import ExperimentPatterns
 from .geometry import EBSDGeometry
 from .knn import ChunkedKNN
@@ -53,6 +54,8 @@ class _ResidentSearch:
         idx = torch.empty((m, k_search), dtype=torch.int32, device=self.device)
         kernels.sim_topk(queries, self.dictionary, k_search, val, idx, index_base=0, merge=False)
         if self.dictionary_f32 is not None:
+            if queries_f32 is None and self.dictionary_f32 is self.dictionary:
+                queries_f32 = queries  # FP32 search: the queries are their own FP32 rows
             if queries_f32 is None:
                 raise ValueError("FP32 rescoring needs the FP32 queries")
             val, idx = kernels.rescore(queries_f32, self.dictionary_f32, idx, topk)
@@ -69,12 +72,21 @@ class DictionaryIndexer(_ResidentSearch):
     an FP32 copy of the dictionary. dict_unit_norm also divides each
     dictionary pattern by its norm (cosine ranking); the reference ranks by
     the raw dot product of zero-mean patterns (False).
+
+    FP32 search (exact_fp32, default): the matrix-core search accumulates
+    each score in one FP32 chain over all D pixels, which leaves errors of
+    up to ~5e-6 of |q||d| at D = 3,600 (a CPU sgemm, with its many partial
+    sums, stays near 2e-7), so its best EBSD_MAX_TOPK candidates are
+    re-scored by the FP32 rescoring kernel (partial sums over lanes, error
+    ~1e-7) and re-ranked: the reported scores and order are those of an
+    accurate FP32 dot product.
     """
 
     def __init__(self, master_pattern: torch.Tensor, rays: torch.Tensor, orientations: torch.Tensor,
                  precision: str = "fp32", rescore_k: int = 0, dict_unit_norm: bool = False,
                  renorm_dirs: bool = False, search_components: Union[int, str] = 0,
-                 explained_variance: float = 0.995, basis_sample: int = 1 << 18, rotate_basis: bool = True):
+                 explained_variance: float = 0.995, basis_sample: int = 1 << 18, rotate_basis: bool = True,
+                 exact_fp32: bool = True):
         if precision not in kernels.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(kernels.PRECISIONS)}")
         if search_components and rescore_k <= 0:
@@ -103,7 +115,12 @@ class DictionaryIndexer(_ResidentSearch):
             return
         self.dictionary = self._project(self.dtype)
         self.dictionary_f32 = None
-        if self.rescore_k > 0 and self.dtype != DTYPE_F32:
+        if self.dtype == DTYPE_F32:
+            if exact_fp32:
+                # the FP32 rows are their own rescoring copy
+                self.dictionary_f32 = self.dictionary
+                self.rescore_k = EBSD_MAX_TOPK
+        elif self.rescore_k > 0:
             self.dictionary_f32 = self._project(DTYPE_F32)
 
     def _build_pca_search(self, components, explained: float, sample: int, chunk: int = 1 << 16):
@@ -175,7 +192,9 @@ class DictionaryIndexer(_ResidentSearch):
         if self.basis_t is None:
             q = kernels.prep_patterns(patterns, self.dtype, zero_mean=zero_mean, unit_norm=unit_norm, pix_idx=pix_idx)
             qf = None
-            if self.dictionary_f32 is not None:
+            if self.dictionary_f32 is self.dictionary:
+                qf = q
+            elif self.dictionary_f32 is not None:
                 qf = kernels.prep_patterns(patterns, DTYPE_F32, zero_mean=zero_mean, unit_norm=unit_norm,
                                            pix_idx=pix_idx)
             return q, qf
@@ -249,9 +268,23 @@ class PCADictionaryIndexer(_ResidentSearch):
         self.n_pix = d
         learn = learn_orientations.to(self.device, torch.float32).contiguous()
         self.covariance = OnlineCovMatrix(d, device=self.device)
+        # stage times (milliseconds) of the build, for measurement: the
+        # learning projection, the covariance updates, the FP64 eigensolve,
+        # and the index dictionary's projection onto the components
+        self.timings = {"learn_projection_ms": 0.0, "covariance_ms": 0.0}
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
         for s in range(0, learn.shape[0], chunk):
-            self.covariance(self._project(learn[s:s + chunk]))
+            ev[0].record()
+            x = self._project(learn[s:s + chunk])
+            ev[1].record()
+            self.covariance(x)
+            ev[2].record()
+            ev[2].synchronize()
+            self.timings["learn_projection_ms"] += ev[0].elapsed_time(ev[1])
+            self.timings["covariance_ms"] += ev[1].elapsed_time(ev[2])
+        t0 = time.perf_counter()
         self.eigenvalues, evecs = self.covariance.get_eigensystem(torch.float64)
+        self.timings["eigensolve_ms"] = (time.perf_counter() - t0) * 1000.0
         if component_slice == "top":
             comps = evecs[:, -n_components:]
         elif component_slice == "reference":
@@ -271,11 +304,23 @@ class PCADictionaryIndexer(_ResidentSearch):
         self.orientations = index.contiguous()
         n_pad = kernels.padded_width(n, DTYPE_F32)
         f32 = torch.empty((index.shape[0], n_pad), dtype=torch.float32, device=self.device)
+        self.timings["index_projection_ms"] = 0.0
+        self.timings["index_coefficients_ms"] = 0.0
         for s in range(0, index.shape[0], chunk):
-            kernels.gemm_nt(self._project(index[s:s + chunk]), pt, out=f32[s:s + chunk])
+            ev[0].record()
+            x = self._project(index[s:s + chunk])
+            ev[1].record()
+            kernels.gemm_nt(x, pt, out=f32[s:s + chunk])
+            ev[2].record()
+            ev[2].synchronize()
+            self.timings["index_projection_ms"] += ev[0].elapsed_time(ev[1])
+            self.timings["index_coefficients_ms"] += ev[1].elapsed_time(ev[2])
         dict_f32 = kernels.PatternMatrix(f32, None, DTYPE_F32, n)
         if self.dtype == DTYPE_F32:
-            self.dictionary, self.dictionary_f32 = dict_f32, None
+            # exact FP32 re-ranking of the matrix-core search's best
+            # candidates on the coefficients (see DictionaryIndexer)
+            self.dictionary, self.dictionary_f32 = dict_f32, dict_f32
+            self.rescore_k = EBSD_MAX_TOPK
         else:
             self.dictionary = kernels.prep_patterns(f32[:, :n], self.dtype, zero_mean=False)
             self.dictionary_f32 = dict_f32 if self.rescore_k > 0 else None
@@ -287,6 +332,15 @@ class PCADictionaryIndexer(_ResidentSearch):
     @property
     def size(self) -> int:
         return self.orientations.shape[0]
+
+    def prepare_pair(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None, zero_mean: bool = True):
+        """(search rows, FP32 coefficient rows for rescoring or None)."""
+        q = self.prepare(patterns, pix_idx, zero_mean)
+        if self.dictionary_f32 is self.dictionary:
+            return q, q
+        if self.dictionary_f32 is not None:
+            return q, self.prepare(patterns, pix_idx, zero_mean, DTYPE_F32)
+        return q, None
 
     def prepare(self, patterns: torch.Tensor, pix_idx: Optional[torch.Tensor] = None,
                 zero_mean: bool = True, dtype: Optional[int] = None) -> kernels.PatternMatrix:
@@ -472,9 +526,7 @@ def pca_dictionary_index_orientations(
         vals, idxs = [], []
         step = max(1, min(n_exp, (4 << 30) // (4 * max(1, exp.shape[1]))))
         for s in range(0, n_exp, step):
-            q = engine.prepare(exp[s:s + step], pix_idx, zero_mean)
-            qf = engine.prepare(exp[s:s + step], pix_idx, zero_mean, DTYPE_F32) \
-                if engine.dictionary_f32 is not None else None
+            q, qf = engine.prepare_pair(exp[s:s + step], pix_idx, zero_mean)
             v, ix = engine.index(q, top_k_matches, qf)
             vals.append(v)
             idxs.append(ix)

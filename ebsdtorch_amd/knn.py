@@ -65,8 +65,30 @@ class ChunkedKNN:
     def _run(self, q: kernels.PatternMatrix, lo: int, hi: int) -> None:
         if self.prepared_data is None:
             raise ValueError("Data chunk is not set.")
-        kernels.sim_topk(q, self.prepared_data, self.topk, self.top_val[lo:hi], self.top_idx[lo:hi],
-                         index_base=self.curr_start, merge=True)
+        if self.dtype != DTYPE_F32:
+            kernels.sim_topk(q, self.prepared_data, self.topk, self.top_val[lo:hi], self.top_idx[lo:hi],
+                             index_base=self.curr_start, merge=True)
+            return
+        # FP32: the chunk's best EBSD_MAX_TOPK candidates from the matrix
+        # cores are re-scored exactly (ebsd_rescore) and merged into the
+        # running lists, ordered by score then index (as the kernels order
+        # ties)
+        k_s = min(EBSD_MAX_TOPK, max(self.topk, self.prepared_data.rows))
+        v = torch.empty((q.rows, k_s), dtype=torch.float32, device=self.match_device)
+        i = torch.empty((q.rows, k_s), dtype=torch.int32, device=self.match_device)
+        kernels.sim_topk(q, self.prepared_data, k_s, v, i, index_base=0, merge=False)
+        k_r = min(self.topk, k_s)
+        v, i = kernels.rescore(q, self.prepared_data, i, k_r)
+        i = torch.where(i >= 0, i + self.curr_start, i)
+        cv = torch.cat([self.top_val[lo:hi], v], 1)
+        ci = torch.cat([self.top_idx[lo:hi], i], 1)
+        # empty slots (index -1, score -inf) sort last
+        key_i = torch.where(ci < 0, torch.iinfo(torch.int32).max, ci)
+        o1 = torch.argsort(key_i, dim=1, stable=True)
+        cv, ci = torch.gather(cv, 1, o1), torch.gather(ci, 1, o1)
+        o2 = torch.argsort(cv, dim=1, descending=True, stable=True)[:, :self.topk]
+        self.top_val[lo:hi] = torch.gather(cv, 1, o2)
+        self.top_idx[lo:hi] = torch.gather(ci, 1, o2)
 
     def query_all(self, query) -> None:
         if self.prepared_data is None:
