@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--topk", type=int, default=16)
     p.add_argument("--mp-size", type=int, default=501)
     p.add_argument("--detector", type=int, default=60)
+    p.add_argument("--agree", type=int, default=1,
+                   help="index the batch (and a held-out set) in plain FP32 too, for agreement (0 = skip, "
+                        "for profiling runs)")
     p.add_argument("--forms", type=int, default=1,
                    help="also measure BF16 and FP8 on full patterns and the library-default search (0 = skip)")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
@@ -66,9 +69,12 @@ def parse():
     p.add_argument("--learned-train", type=int, default=2_000_000,
                    help="training patterns of the learned-proposal form (0 = skip that form)")
     p.add_argument("--learned-batch", type=int, default=65536, help="patterns indexed by the learned form")
-    p.add_argument("--workload", default="metric", choices=["metric", "c3", "c5"],
+    p.add_argument("--workload", default="metric", choices=["metric", "c3", "c5", "projection"],
                    help="metric: the headline configuration; c3: the precision forms and PCA DI at 80x80; "
                         "c5: 6/mmm, 120x120, 1M dictionary, streamed scan")
+    p.add_argument("--proj-config", default="metric", choices=["metric", "c3", "c5"],
+                   help="projection workload: metric (m-3m, 60x60), c3 (m-3m, 80x80) or c5 (6/mmm, 60 families, "
+                        "120x120)")
     p.add_argument("--c3-pca-components", type=int, default=1000, help="c3: components of the PCA DI form")
     p.add_argument("--c3-learn", type=int, default=1 << 20,
                    help="c3: learning-dictionary rows for the PCA DI covariance (first rows of the dictionary)")
@@ -99,6 +105,9 @@ def main():
         return
     if a.workload == "c3":
         print(json.dumps(c3_workload(a, dev)), flush=True)
+        return
+    if a.workload == "projection":
+        print(json.dumps(projection_workload(a, dev)), flush=True)
         return
     if a.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
@@ -157,6 +166,12 @@ def main():
     value = head["rate"]
     log(f"{a.precision}: {head['ms_per_step']:.1f} ms/step, similarity {head['sim_ms']:.1f} ms, {value:.0f} patterns/s")
 
+    if not a.agree:
+        res = {"metric": "patterns indexed/s (profiling run: no agreement measured)", "value": round(value, 1),
+               "unit": "patterns/s", "ms_per_step": round(head["ms_per_step"], 3),
+               "similarity_avg_launch_ms": round(head["sim_ms"], 3), "projection_ms": round(proj_kernel_ms, 3)}
+        print(json.dumps(res), flush=True)
+        return
     # ---- plain FP32 DI over the whole batch: agreement, and its own rate ----
     ref = E.DictionaryIndexer(mp, rays, dico, precision="fp32")
     fp32 = time_form(ref, exp, a.topk, 1, 0, warmup_rows=2048)
@@ -756,6 +771,55 @@ def learned_form(a, mp, dico, dev):
     log(f"learned: {rate:.0f} patterns/s vs FP32 DI {fp32_rate:.0f}/s, fallback {fallback_rate:.4f}, "
         f"err quantiles {quant}, local kernel {local_ms:.1f} ms, rescore kernel {rescore_ms:.1f} ms")
     return out
+
+
+def projection_workload(a, dev):
+    """The projection kernel alone (for counters): the metric dictionary
+    (995,277 m-3m orientations, 60x60), the same at 80x80 (C3) or the C5 one
+    (986,945 6/mmm orientations, 120x120, 60 band families), projected with zero mean into
+    FP32 rows and into FP8 rows (per-row power-of-two scale), `steps` times
+    each after one warm-up; HIP-event times and algorithmic GB/s (output rows
+    + 16 bytes of quaternion per orientation)."""
+    import torch
+
+    import ebsdtorch_amd as E
+    from ebsdtorch_amd import kernels, synthetic
+    from ebsdtorch_amd._lib import DTYPE_F32, DTYPE_FP8_E4M3
+
+    laue, det, fam, seed = {"metric": (11, 60, 40, 0), "c3": (11, 80, 40, 0), "c5": (9, 120, 60, 10)}[a.proj_config]
+    mp = synthetic.band_master_pattern(a.mp_size, laue, fam, 2.0, seed=seed, device=dev)
+    dico = E.sample_ori_fz_laue(laue, a.dict_size, dev, permute=True,
+                                generator=torch.Generator().manual_seed(1 if laue == 11 else 11))
+    rays = E.EBSDGeometry((det, det), proj_center=(0.5, 0.5, 0.6)).get_rays().to(dev)
+    quads = kernels.mp_quad_table(mp)
+    D = det * det
+    out = {}
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
+    for name, dt, eb in (("fp32", DTYPE_F32, 4), ("fp8", DTYPE_FP8_E4M3, 1)):
+        o = kernels.alloc_patterns(dico.shape[0], D, dt, dev)
+        kernels.project(mp, rays, dico, dt, out=o, quads=quads)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(a.steps):
+            kernels.project(mp, rays, dico, dt, out=o, quads=quads)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / a.steps
+        nbytes = dico.shape[0] * (o.ld * eb + (4 if dt == DTYPE_FP8_E4M3 else 0) + 16)
+        out[name] = {"ms": round(ms, 3), "GBps": round(nbytes / ms / 1e6, 1),
+                     "hbm_frac": round(nbytes / ms / 1e6 / PEAKS["hbm_GBps"], 4), "bytes": nbytes,
+                     "pixels_per_s": round(dico.shape[0] * D / ms * 1000, 1)}
+        log(f"projection {a.proj_config} {name}: {ms:.2f} ms, {out[name]['GBps']} GB/s")
+        del o
+    return {"workload": f"projection of {dico.shape[0]} orientations onto {det}x{det} "
+                        f"({'m-3m' if laue == 11 else '6/mmm'} {a.mp_size}x{a.mp_size} master pattern)",
+            "steps": a.steps, **out}
+
 
 
 def c3_workload(a, dev):

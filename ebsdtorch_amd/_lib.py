@@ -15,7 +15,7 @@ import torch
 
 from ._build import LIB_PATH
 
-EBSD_ABI_VERSION = 1
+EBSD_ABI_VERSION = 2
 
 EBSD_OK = 0
 EBSD_ERR_NULL = -1
@@ -60,7 +60,9 @@ _SIGNATURES = {
     ),
     "ebsd_project_quads": (
         _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int,
-                 _c_void_p, _c_int, _c_void_p]),
+                 _c_void_p, _c_int, _c_void_p, _c_void_p]),
+    "ebsd_view_keys": (
+        _c_int, [_c_void_p, _c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _c_void_p, _c_void_p]),
     "ebsd_project_shifted": (
         _c_int,
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_void_p,

@@ -60,10 +60,13 @@ __device__ __forceinline__ float block_sum(float v, float* scratch) {
     float p[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) p[i] = scratch[i];
+    // pairwise levels; an odd element is carried to the next level
 #pragma unroll
-    for (int n = W / 2; n > 0; n >>= 1)
+    for (int len = W; len > 1; len = (len + 1) / 2) {
 #pragma unroll
-      for (int i = 0; i < n; ++i) p[i] = p[2 * i] + p[2 * i + 1];
+      for (int i = 0; i < len / 2; ++i) p[i] = p[2 * i] + p[2 * i + 1];
+      if (len & 1) p[len / 2] = p[len - 1];
+    }
     return p[0];
   }
 }

@@ -120,6 +120,33 @@ __device__ __forceinline__ float sample_border_quad(const float4* __restrict__ q
   return ((t.x * (s * e) + t.y * (s * w)) + t.z * (n * e)) + t.w * (n * w);
 }
 
+// Source position of a bilinear, border, align_corners=False sample split
+// into the quad-table texel (y0 * W + x0) and the two fractional offsets:
+// the arithmetic of sample_border_quad, so quad_blend(quads[texel], w, n)
+// equals sample_border_quad(quads, H, W, gx, gy) bit for bit. The clip to
+// [0, size - 1] is one v_med3 (the same value as fmin(fmax(., 0), size - 1)
+// for the finite coordinates the Lambert map produces).
+__device__ __forceinline__ int quad_source(int H, int W, float gx, float gy, float& w, float& n) {
+#pragma clang fp contract(off)
+  float ix = (gx + 1.0f) * (0.5f * (float)W) - 0.5f;
+  float iy = (gy + 1.0f) * (0.5f * (float)H) - 0.5f;
+  ix = __builtin_amdgcn_fmed3f(ix, 0.0f, (float)(W - 1));
+  iy = __builtin_amdgcn_fmed3f(iy, 0.0f, (float)(H - 1));
+  const float x0f = floorf(ix);
+  const float y0f = floorf(iy);
+  w = ix - x0f;
+  n = iy - y0f;
+  return (int)y0f * W + (int)x0f;
+}
+
+// The blend of sample_border_quad for a fetched texel t.
+__device__ __forceinline__ float quad_blend(const float4 t, float w, float n) {
+#pragma clang fp contract(off)
+  const float e = 1.0f - w;
+  const float s = 1.0f - n;
+  return ((t.x * (s * e) + t.y * (s * w)) + t.z * (n * e)) + t.w * (n * w);
+}
+
 // x / |x| with the squared norm accumulated as fma(z, z, fma(y, y, x*x)),
 // the rounding sequence of torch's float32 CPU vector norm over a length-3
 // row (master_pattern.py:166-167).

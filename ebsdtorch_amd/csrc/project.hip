@@ -7,9 +7,10 @@
 // and, per orientation, the mean subtraction of indexing.py:139-143 plus the
 // optional unit norm / low-precision cast used by the precision forms.
 //
-// One 256-thread workgroup owns one pattern. Its pixels live in registers
-// (ITEMS per thread) between the sampling pass and the normalisation pass, so
-// every output byte is written exactly once and coalesced. The master pattern
+// One workgroup (256 threads up to 4,096 pixels, 1,024 above) owns one
+// pattern. Its pixels live in registers (at most 16 per thread) between the
+// sampling pass and the normalisation pass, so every output byte is written
+// exactly once and coalesced. The master pattern
 // (2 x S^2 floats, 2 MB at S = 501) stays resident in each XCD's L2.
 //
 // Numerics: rotation, Lambert arithmetic and the bilinear blend round step by
@@ -97,36 +98,123 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const int d = tid + i * BLOCK;
+#ifdef EBSD_NT_STORE
+    if (d < ld) __builtin_nontemporal_store(to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f), orow + d);
+#else
     if (d < ld) orow[d] = to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f);
+#endif
   }
   // Columns beyond ITEMS*BLOCK (only when ld exceeds it) are zero padding.
   for (int d = ITEMS * BLOCK + tid; d < ld; d += BLOCK) orow[d] = to_out<OutT>(0.0f);
 }
 
-// Persistent: workgroup g projects patterns g, g + gridDim.x, ... With at
-// most 16 pixels per thread the thread's rays stay in registers across
-// patterns; larger detectors re-read them (L1/L2 resident) per pattern.
-template <int ITEMS, typename OutT>
-__global__ __launch_bounds__(256) void project_kernel(
+// Sample-frame direction of pixel slot d of pattern n (rotation by the
+// pattern's quaternion q; EXTRA: the per-pattern pixel shift o with the
+// re-normalisation of the shifted ray, and/or the re-normalisation of the
+// rotated one), then its side-by-side Lambert position.
+template <bool EXTRA>
+__device__ __forceinline__ void pixel_position(float4 q, float bx, float by, float bz, bool shift, float ox,
+                                               float oy, float oz, bool renorm, float& gx, float& gy) {
+  if constexpr (EXTRA) {
+    if (shift) {
+      bx += ox;
+      by += oy;
+      bz += oz;
+      renormalise(bx, by, bz);
+    }
+  }
+  float x, y, z;
+  rotate_ray(q.x, q.y, q.z, q.w, bx, by, bz, x, y, z);
+  if constexpr (EXTRA) {
+    if (renorm) renormalise(x, y, z);
+  }
+  lambert_sbs(x, y, z, gx, gy);
+}
+
+// The ITEMS values of one pattern held by this thread (slots d = tid + i *
+// BLOCK; slots past n_pix read 0). Per group of kProjGroup slots, the
+// positions and texel addresses are computed first, then the gathers are
+// issued, then the blends, with no branch between slots, so the gathers of
+// consecutive slots overlap. Measured on the 995,277-orientation
+// dictionaries (FP32 rows): groups of 1 / 2 / 4 / 8 / 16 slots take 16.7 /
+// 15.0 / 15.2 / 16.6 / 21.7 ms at 60 x 60 and 85.9 / 70.3 / 74.7 / 100.5 /
+// 134.8 ms at 120 x 120 (larger groups hold more registers and lower the
+// occupancy); one branchy slot at a time took 14.9 / 121 ms.
+constexpr int kProjGroup = 2;
+
+template <int ITEMS, int BLOCK, bool QUADS, bool EXTRA>
+__device__ __forceinline__ void pattern_values(float (&vals)[ITEMS], const float* __restrict__ mp,
+                                               const float4* __restrict__ quads, int H, int W,
+                                               const float (&rx)[ITEMS], const float (&ry)[ITEMS],
+                                               const float (&rz)[ITEMS], int n_pix, float4 q, bool shift,
+                                               float ox, float oy, float oz, bool renorm) {
+  constexpr int G = ITEMS < kProjGroup ? ITEMS : kProjGroup;
+#pragma unroll
+  for (int g0 = 0; g0 < ITEMS; g0 += G) {
+    // the last group may be short (ITEMS not a multiple of G)
+    constexpr int kFull = G;
+    const int n_in = ITEMS - g0 < kFull ? ITEMS - g0 : kFull;  // compile-time after unrolling
+    float gx[G], gy[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < n_in)
+        pixel_position<EXTRA>(q, rx[g0 + j], ry[g0 + j], rz[g0 + j], shift, ox, oy, oz, renorm, gx[j], gy[j]);
+    if constexpr (QUADS) {
+      int t[G];
+      float fw[G], fn[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j)
+        if (j < n_in) t[j] = quad_source(H, W, gx[j], gy[j], fw[j], fn[j]);
+      float4 tap[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j)
+        if (j < n_in) tap[j] = quads[t[j]];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        if (j < n_in) {
+          const int d = threadIdx.x + (g0 + j) * BLOCK;
+          vals[g0 + j] = d < n_pix ? quad_blend(tap[j], fw[j], fn[j]) : 0.0f;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        if (j < n_in) {
+          const int d = threadIdx.x + (g0 + j) * BLOCK;
+          vals[g0 + j] = d < n_pix ? sample_border(mp, H, W, gx[j], gy[j]) : 0.0f;
+        }
+      }
+    }
+  }
+}
+
+// Persistent: workgroup g projects patterns g, g + gridDim.x, ... One
+// workgroup of BLOCK threads per pattern, ITEMS (<= 16) pixel slots per
+// thread; the thread's rays stay in registers across patterns. The optional
+// per-pattern shift and re-normalisations (uniform flags) select a second
+// copy of the pixel loop, so the plain path carries no branch per pixel.
+template <int ITEMS, int BLOCK, typename OutT>
+__global__ __launch_bounds__(BLOCK) void project_kernel(
     const float* __restrict__ mp, const float4* __restrict__ quads, int H, int W,
     const float* __restrict__ rays, int n_pix, const float* __restrict__ quats, int n_ori,
     OutT* __restrict__ out, int ld, float* __restrict__ out_scale, int flags,
-    const float* __restrict__ offsets) {
-  __shared__ float red[4];
-  constexpr bool kRaysInRegs = ITEMS <= 16;
+    const float* __restrict__ offsets, const int* __restrict__ order) {
+  __shared__ float red[BLOCK / 64];
   const bool renorm = (flags & EBSD_FLAG_RENORM_DIRS) != 0;
-  float rx[kRaysInRegs ? ITEMS : 1], ry[kRaysInRegs ? ITEMS : 1], rz[kRaysInRegs ? ITEMS : 1];
-  if constexpr (kRaysInRegs) {
+  const bool extra = renorm || offsets != nullptr;
+  float rx[ITEMS], ry[ITEMS], rz[ITEMS];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const int d = threadIdx.x + i * 256;
-      const int dd = d < n_pix ? d : 0;
-      rx[i] = __ldg(rays + 3 * dd + 0);
-      ry[i] = __ldg(rays + 3 * dd + 1);
-      rz[i] = __ldg(rays + 3 * dd + 2);
-    }
+  for (int i = 0; i < ITEMS; ++i) {
+    const int d = threadIdx.x + i * BLOCK;
+    const int dd = d < n_pix ? d : n_pix - 1;  // padding slots repeat a real ray (value dropped)
+    rx[i] = __ldg(rays + 3 * dd + 0);
+    ry[i] = __ldg(rays + 3 * dd + 1);
+    rz[i] = __ldg(rays + 3 * dd + 2);
   }
-  for (int n = blockIdx.x; n < n_ori; n += gridDim.x) {
+  for (int k = blockIdx.x; k < n_ori; k += gridDim.x) {
+    // processing slot k projects row `order[k]` (a permutation of the rows),
+    // or row k: the order only decides which patterns run together
+    const int n = order ? order[k] : k;
     const float4 q = *reinterpret_cast<const float4*>(quats + 4 * (size_t)n);
     // per-pattern shift of the (un-normalised) pixel positions: a moved
     // pattern centre translates every detector pixel by the same vector
@@ -137,37 +225,22 @@ __global__ __launch_bounds__(256) void project_kernel(
       oz = offsets[3 * (size_t)n + 2];
     }
     float vals[ITEMS];
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const int d = threadIdx.x + i * 256;
-      float v = 0.0f;
-      if (d < n_pix) {
-        float bx, by, bz;
-        if constexpr (kRaysInRegs) {
-          bx = rx[i];
-          by = ry[i];
-          bz = rz[i];
-        } else {
-          bx = __ldg(rays + 3 * d + 0);
-          by = __ldg(rays + 3 * d + 1);
-          bz = __ldg(rays + 3 * d + 2);
-        }
-        if (offsets) {
-          bx += ox;
-          by += oy;
-          bz += oz;
-          renormalise(bx, by, bz);
-        }
-        float x, y, z;
-        rotate_ray(q.x, q.y, q.z, q.w, bx, by, bz, x, y, z);
-        if (renorm) renormalise(x, y, z);
-        float gx, gy;
-        lambert_sbs(x, y, z, gx, gy);
-        v = quads ? sample_border_quad(quads, H, W, gx, gy) : sample_border(mp, H, W, gx, gy);
-      }
-      vals[i] = v;
+    if (quads) {
+      if (extra)
+        pattern_values<ITEMS, BLOCK, true, true>(vals, mp, quads, H, W, rx, ry, rz, n_pix, q, offsets != nullptr,
+                                                 ox, oy, oz, renorm);
+      else
+        pattern_values<ITEMS, BLOCK, true, false>(vals, mp, quads, H, W, rx, ry, rz, n_pix, q, false, 0.0f, 0.0f,
+                                                  0.0f, false);
+    } else {
+      if (extra)
+        pattern_values<ITEMS, BLOCK, false, true>(vals, mp, quads, H, W, rx, ry, rz, n_pix, q, offsets != nullptr,
+                                                  ox, oy, oz, renorm);
+      else
+        pattern_values<ITEMS, BLOCK, false, false>(vals, mp, quads, H, W, rx, ry, rz, n_pix, q, false, 0.0f,
+                                                   0.0f, 0.0f, false);
     }
-    finish_row<ITEMS, OutT>(vals, n_pix, flags, out, ld, out_scale, (size_t)n, red);
+    finish_row<ITEMS, OutT, BLOCK>(vals, n_pix, flags, out, ld, out_scale, (size_t)n, red);
   }
 }
 
@@ -236,23 +309,28 @@ __global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restric
 template <typename OutT>
 int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, const float* quats,
                    int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st,
-                   const float* offsets = nullptr, const float4* quads = nullptr) {
-  // ~16 resident workgroups per CU on 256 CUs; each walks n_ori / grid patterns
-  const dim3 grid(n_ori < 4096 ? n_ori : 4096), block(256);
+                   const float* offsets = nullptr, const float4* quads = nullptr, const int* order = nullptr) {
   OutT* o = static_cast<OutT*>(out);
-  const int items = (n_pix + 255) / 256;
-#define EBSD_PROJ_CASE(I)                                                                        \
-  if (items <= I) {                                                                              \
-    hipLaunchKernelGGL((project_kernel<I, OutT>), grid, block, 0, st, mp, quads, H, W, rays, n_pix,  \
-                       quats, n_ori, o, ld, out_scale, flags, offsets);                            \
-    EBSD_LAUNCH_CHECK();                                                                         \
-    return EBSD_OK;                                                                              \
+  // the first (slots per thread, threads) pair that covers n_pix. Pairs that
+  // fit the detector more tightly (60 x 60 on 19 x 192 slots, 80 x 80 on
+  // 20 x 320, 120 x 120 on 15 x 960) spend less VALU work on padding slots
+  // but hold more registers per thread and ran slower (15.5 / 35.5 / 78.8 ms
+  // against 15.0 / 30.6 / 70.3 ms for the FP32 rows of the 1M dictionaries)
+  // persistent: about 4,096 x 256 threads (16 workgroups of 256 per CU on
+  // 256 CUs); each workgroup walks n_ori / grid patterns
+#define EBSD_PROJ_CASE(I, B)                                                                            \
+  if (n_pix <= (I) * (B)) {                                                                             \
+    const int g = (4096 * 256) / (B);                                                                   \
+    hipLaunchKernelGGL((project_kernel<I, B, OutT>), dim3(n_ori < g ? n_ori : g), dim3(B), 0, st, mp,   \
+                       quads, H, W, rays, n_pix, quats, n_ori, o, ld, out_scale, flags, offsets, order); \
+    EBSD_LAUNCH_CHECK();                                                                                \
+    return EBSD_OK;                                                                                     \
   }
-  EBSD_PROJ_CASE(4)
-  EBSD_PROJ_CASE(8)
-  EBSD_PROJ_CASE(16)
-  EBSD_PROJ_CASE(32)
-  EBSD_PROJ_CASE(64)
+  EBSD_PROJ_CASE(4, 256)
+  EBSD_PROJ_CASE(8, 256)
+  EBSD_PROJ_CASE(16, 256)
+  EBSD_PROJ_CASE(8, 1024)
+  EBSD_PROJ_CASE(16, 1024)
 #undef EBSD_PROJ_CASE
   return EBSD_ERR_SHAPE;
 }
@@ -344,7 +422,7 @@ extern "C" int ebsd_project(const float* mp, int mp_rows, int mp_cols, const flo
 
 extern "C" int ebsd_project_quads(const float* quads, int mp_rows, int mp_cols, const float* rays, int n_pix,
                                   const float* quats, int n_ori, void* out, int out_dtype, int ld_out,
-                                  float* out_scale, int flags, void* stream) {
+                                  float* out_scale, int flags, const int* order, void* stream) {
   if (!quads || !rays || !quats || !out) return EBSD_ERR_NULL;
   if (mp_rows < 1 || mp_cols < 1 || n_ori < 0) return EBSD_ERR_SHAPE;
   if ((int64_t)mp_rows * mp_cols > (int64_t)INT_MAX) return EBSD_ERR_SHAPE;
@@ -358,18 +436,58 @@ extern "C" int ebsd_project_quads(const float* quads, int mp_rows, int mp_cols, 
   switch (out_dtype) {
     case EBSD_DTYPE_F32:
       return launch_project<float>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale,
-                                   flags, st, nullptr, q4);
+                                   flags, st, nullptr, q4, order);
     case EBSD_DTYPE_BF16:
       return launch_project<__hip_bfloat16>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out,
-                                            out_scale, flags, st, nullptr, q4);
+                                            out_scale, flags, st, nullptr, q4, order);
     case EBSD_DTYPE_F16:
       return launch_project<__half>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale,
-                                    flags, st, nullptr, q4);
+                                    flags, st, nullptr, q4, order);
     case EBSD_DTYPE_FP8_E4M3:
       return launch_project<uint8_t>(nullptr, mp_rows, mp_cols, rays, n_pix, quats, n_ori, out, ld_out, out_scale,
-                                     flags, st, nullptr, q4);
+                                     flags, st, nullptr, q4, order);
   }
   return EBSD_ERR_DTYPE;
+}
+
+namespace {
+
+// Interleave the low 7 bits of x with the low 6 bits of y (y bits above x's).
+__device__ __forceinline__ int morton_7x6(int x, int y) {
+  int k = 0;
+#pragma unroll
+  for (int b = 0; b < 7; ++b) {
+    k |= ((x >> b) & 1) << (2 * b);
+    if (b < 6) k |= ((y >> b) & 1) << (2 * b + 1);
+  }
+  return k;
+}
+
+__global__ __launch_bounds__(256) void view_keys_kernel(const float* __restrict__ quats, int n, float dx, float dy,
+                                                        float dz, int* __restrict__ keys) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = *reinterpret_cast<const float4*>(quats + 4 * (size_t)i);
+  float x, y, z, gx, gy;
+  rotate_ray(q.x, q.y, q.z, q.w, dx, dy, dz, x, y, z);
+  lambert_sbs(x, y, z, gx, gy);
+  const int cx = min(max((int)((gx + 1.0f) * 64.0f), 0), 127);  // 128 columns over the 2S-wide image
+  const int cy = min(max((int)((gy + 1.0f) * 32.0f), 0), 63);   // 64 rows
+  keys[i] = morton_7x6(cx, cy);
+}
+
+}  // namespace
+
+extern "C" int ebsd_view_keys(const float* quats, int n_ori, float dir_x, float dir_y, float dir_z, int* keys,
+                              void* stream) {
+  if (!quats || !keys) return EBSD_ERR_NULL;
+  if (n_ori < 0) return EBSD_ERR_SHAPE;
+  if ((reinterpret_cast<uintptr_t>(quats) & 15) != 0) return EBSD_ERR_ALIGN;
+  if (n_ori == 0) return EBSD_OK;
+  hipLaunchKernelGGL(view_keys_kernel, dim3((n_ori + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     quats, n_ori, dir_x, dir_y, dir_z, keys);
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
 }
 
 extern "C" int ebsd_project_shifted(const float* mp, int mp_rows, int mp_cols, const float* coords,

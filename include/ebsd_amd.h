@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EBSD_ABI_VERSION 1
+#define EBSD_ABI_VERSION 2
 
 /* status codes */
 #define EBSD_OK 0
@@ -90,12 +90,27 @@ int ebsd_project(const float* mp, int mp_rows, int mp_cols, const float* rays,
  * (ebsd_mp_quad_table, dtype F32): one 16-byte load per bilinear sample
  * instead of four, same taps and blend, so the output equals ebsd_project's
  * bit for bit. quads (mp_rows, mp_cols, 4) f32, 16-byte aligned.
+ * order: NULL, or a permutation of [0, n_ori) (int32, device) giving the
+ * order in which rows are processed; it changes nothing in the output, only
+ * which patterns run together (rows sorted by ebsd_view_keys read the same
+ * master-pattern region at the same time, which keeps the gathers in L2).
  * Replaces: as ebsd_project.
  */
 int ebsd_project_quads(const float* quads, int mp_rows, int mp_cols,
                        const float* rays, int n_pix, const float* quats,
                        int n_ori, void* out, int out_dtype, int ld_out,
-                       float* out_scale, int flags, void* stream);
+                       float* out_scale, int flags, const int* order, void* stream);
+
+/*
+ * Locality key of each orientation for ebsd_project_quads' processing order:
+ * keys[n] = Z-order index of the 128 x 64 cell of the side-by-side Lambert
+ * image where direction (dir_x, dir_y, dir_z) (the detector's central ray)
+ * lands after rotation by quats[n].
+ *   quats (n_ori, 4) f32, 16-byte aligned; keys (n_ori,) int32
+ * No counterpart in the reference (scheduling only).
+ */
+int ebsd_view_keys(const float* quats, int n_ori, float dir_x, float dir_y, float dir_z,
+                   int* keys, void* stream);
 
 /*
  * ebsd_project with a per-orientation pattern-centre shift: ray d of row n

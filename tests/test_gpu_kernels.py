@@ -51,6 +51,31 @@ def test_projection_matches_oracle(mp101, shape, n):
     assert (g[:, : rays.shape[0]] - ref).abs().max().item() < 2e-5
 
 
+@pytest.mark.parametrize("shape,dtype", [((60, 60), DTYPE_F32), ((60, 60), DTYPE_FP8_E4M3), ((120, 120), DTYPE_F32),
+                                         ((20, 20), DTYPE_BF16)])
+def test_projection_processing_order_changes_nothing(mp101, shape, dtype):
+    """The processing order of ebsd_project_quads (view_order's sort, any
+    permutation, or none) only schedules rows: the output is bit for bit the
+    same, and equals the plain ebsd_project's."""
+    _say(f"projection order {shape} dtype={dtype}")
+    mp = mp101.to(DEV)
+    rays = _rays(shape).to(DEV)
+    q = synthetic.random_orientations(3000, seed=13).to(DEV)
+    quads = kernels.mp_quad_table(mp)
+    plain = kernels.project(mp, rays, q, dtype)
+    base = kernels.project(mp, rays, q, dtype, quads=quads, order=None)
+    order = kernels.view_order(q, rays)
+    assert order.dtype == torch.int32 and torch.equal(torch.sort(order.long()).values,
+                                                      torch.arange(3000, device=DEV))
+    perm = torch.randperm(3000, generator=torch.Generator().manual_seed(4)).to(DEV, torch.int32)
+    for o in (order, perm):
+        got = kernels.project(mp, rays, q, dtype, quads=quads, order=o)
+        assert torch.equal(got.data, base.data)
+        if dtype == DTYPE_FP8_E4M3:
+            assert torch.equal(got.scale, base.scale)
+    assert torch.equal(base.data, plain.data)
+
+
 def test_projection_raw_unit_norm_and_renorm(mp101):
     _say("projection flags")
     rays = _rays()
