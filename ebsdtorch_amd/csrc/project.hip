@@ -98,11 +98,7 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const int d = tid + i * BLOCK;
-#ifdef EBSD_NT_STORE
-    if (d < ld) __builtin_nontemporal_store(to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f), orow + d);
-#else
     if (d < ld) orow[d] = to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f);
-#endif
   }
   // Columns beyond ITEMS*BLOCK (only when ld exceeds it) are zero padding.
   for (int d = ITEMS * BLOCK + tid; d < ld; d += BLOCK) orow[d] = to_out<OutT>(0.0f);
