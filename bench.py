@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--agree", type=int, default=1,
                    help="index the batch (and a held-out set) in plain FP32 too, for agreement (0 = skip, "
                         "for profiling runs)")
+    p.add_argument("--agree-sample", type=int, default=None,
+                   help="0: the same as --agree 0 --forms 0 (a profiling run of the headline form alone)")
     p.add_argument("--forms", type=int, default=1,
                    help="also measure BF16 and FP8 on full patterns and the library-default search (0 = skip)")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
@@ -82,7 +84,10 @@ def parse():
     p.add_argument("--c5-train", type=int, default=2_000_000, help="c5: training patterns of the 6/mmm network")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run a 1 GiB device copy (reference kernel for PMC byte counters)")
-    return p.parse_args()
+    a = p.parse_args()
+    if a.agree_sample == 0:
+        a.agree, a.forms = 0, 0
+    return a
 
 
 def main():
@@ -166,52 +171,49 @@ def main():
     value = head["rate"]
     log(f"{a.precision}: {head['ms_per_step']:.1f} ms/step, similarity {head['sim_ms']:.1f} ms, {value:.0f} patterns/s")
 
-    if not a.agree:
-        res = {"metric": "patterns indexed/s (profiling run: no agreement measured)", "value": round(value, 1),
-               "unit": "patterns/s", "ms_per_step": round(head["ms_per_step"], 3),
-               "similarity_avg_launch_ms": round(head["sim_ms"], 3), "projection_ms": round(proj_kernel_ms, 3)}
-        print(json.dumps(res), flush=True)
-        return
-    # ---- plain FP32 DI over the whole batch: agreement, and its own rate ----
-    ref = E.DictionaryIndexer(mp, rays, dico, precision="fp32")
-    fp32 = time_form(ref, exp, a.topk, 1, 0, warmup_rows=2048)
-    q32 = ref.prepare(exp)
-    agree = float((fp32["idx"][:, 0] == head["idx"][:, 0]).float().mean())
-    head_mis = mismatch_report(q32, ref.dictionary, head["idx"][:, 0], fp32["idx"][:, 0])
-    log(f"top-1 agreement with FP32 over all {a.batch} patterns: {agree:.5f}; FP32 DI {fp32['rate']:.0f} patterns/s")
-    # held out: orientations and noise the component count was not chosen on
-    ho_n = min(16384, a.batch)
-    ho_ori = synthetic.random_orientations(ho_n, seed=202, device=dev)
-    ho_exp = synthetic.simulate_patterns(mp, rays, ho_ori, mean_counts=200.0, gaussian_sigma=0.05, seed=203)
-    ho_agree = float((form_top1(ref, ho_exp) == form_top1(eng, ho_exp)).float().mean())
-    log(f"held-out top-1 agreement ({ho_n} patterns, orientation seed 202, noise seed 203): {ho_agree:.5f}")
-
     flops = 2.0 * a.batch * n_dict * search_dim
     achieved = flops / (head["sim_ms"] / 1000) / 1e12
     peak = PEAKS[a.precision]
     out_bytes = 4 if pca_search else {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}[a.precision]
     proj_bytes = n_dict * (D * out_bytes + 16) + 2 * a.mp_size * a.mp_size * 4 + 12 * D
     proj_gbps = proj_bytes / (proj_kernel_ms / 1000) / 1e9
-    head_cos = {"reported_top1": cos_error(q32, ref.dictionary, head["idx"][:, 0], head["val"][:, 0])}
-    if head["search_idx"] is not None:
-        head_cos["search_stage_top1"] = cos_error(q32, ref.dictionary, head["search_idx"][:, 0],
-                                                  head["search_val"][:, 0])
-    fp32_cos = {"reported_top1": cos_error(q32, ref.dictionary, fp32["idx"][:, 0], fp32["val"][:, 0]),
-                "search_stage_top1": cos_error(q32, ref.dictionary, fp32["search_idx"][:, 0],
-                                               fp32["search_val"][:, 0])}
 
+    # ---- plain FP32 DI over the whole batch: agreement, and its own rate ----
+    agree = head_mis = fp32 = None
+    ho_n, ho_agree = min(16384, a.batch), None
+    head_cos = fp32_cos = None
     forms = {}
-    if a.forms and a.precision != "fp32":
-        del eng
-        torch.cuda.empty_cache()
-        for name, prec, comps, ev in (("bf16_full_patterns", "bf16", 0, None),
-                                      ("fp8_full_patterns", "fp8", 0, None),
-                                      ("fp8_pca_library_default", "fp8", "auto", 0.995)):
-            forms[name] = sub_form(mp, rays, dico, exp, ho_exp, ref, q32, fp32, prec, comps, ev, a)
+    if a.agree:
+        ref = E.DictionaryIndexer(mp, rays, dico, precision="fp32")
+        fp32 = time_form(ref, exp, a.topk, 1, 0, warmup_rows=2048)
+        q32 = ref.prepare(exp)
+        agree = float((fp32["idx"][:, 0] == head["idx"][:, 0]).float().mean())
+        head_mis = mismatch_report(q32, ref.dictionary, head["idx"][:, 0], fp32["idx"][:, 0])
+        log(f"top-1 agreement with FP32 over all {a.batch} patterns: {agree:.5f}; FP32 DI {fp32['rate']:.0f} "
+            "patterns/s")
+        # held out: orientations and noise the component count was not chosen on
+        ho_ori = synthetic.random_orientations(ho_n, seed=202, device=dev)
+        ho_exp = synthetic.simulate_patterns(mp, rays, ho_ori, mean_counts=200.0, gaussian_sigma=0.05, seed=203)
+        ho_agree = float((form_top1(ref, ho_exp) == form_top1(eng, ho_exp)).float().mean())
+        log(f"held-out top-1 agreement ({ho_n} patterns, orientation seed 202, noise seed 203): {ho_agree:.5f}")
+        head_cos = {"reported_top1": cos_error(q32, ref.dictionary, head["idx"][:, 0], head["val"][:, 0])}
+        if head["search_idx"] is not None:
+            head_cos["search_stage_top1"] = cos_error(q32, ref.dictionary, head["search_idx"][:, 0],
+                                                      head["search_val"][:, 0])
+        fp32_cos = {"reported_top1": cos_error(q32, ref.dictionary, fp32["idx"][:, 0], fp32["val"][:, 0]),
+                    "search_stage_top1": cos_error(q32, ref.dictionary, fp32["search_idx"][:, 0],
+                                                   fp32["search_val"][:, 0])}
+        if a.forms and a.precision != "fp32":
+            del eng
             torch.cuda.empty_cache()
-        eng = None
-    del ref, q32
-    torch.cuda.empty_cache()
+            for name, prec, comps, ev in (("bf16_full_patterns", "bf16", 0, None),
+                                          ("fp8_full_patterns", "fp8", 0, None),
+                                          ("fp8_pca_library_default", "fp8", "auto", 0.995)):
+                forms[name] = sub_form(mp, rays, dico, exp, ho_exp, ref, q32, fp32, prec, comps, ev, a)
+                torch.cuda.empty_cache()
+            eng = None
+        del ref, q32
+        torch.cuda.empty_cache()
 
     refinement = None
     if a.refine:
@@ -262,19 +264,19 @@ def main():
             "search": search_cfg,
         },
         "top1_agreement_vs_fp32": agree,
-        "agreement_sample": a.batch,
+        "agreement_sample": None if agree is None else a.batch,
         "top1_disagreements_vs_fp32": head_mis,
         "heldout_top1_agreement_vs_fp32": {"value": ho_agree, "patterns": ho_n,
                                            "seeds": "orientations 202, noise 203 (the batch: 2, 3)"},
-        "fp32_di_patterns_per_s": round(fp32["rate"], 1),
-        "speedup_vs_fp32_di": round(value / fp32["rate"], 2),
+        "fp32_di_patterns_per_s": None if fp32 is None else round(fp32["rate"], 1),
+        "speedup_vs_fp32_di": None if fp32 is None else round(value / fp32["rate"], 2),
         "including_dictionary_build": {
             "build_s": round(build_s, 3),
             "patterns_per_s_one_batch": round(a.batch / (build_s + head["ms_per_step"] / 1000), 1),
             "what": "one batch indexed right after building the dictionary engine (projection, second moment, "
                     "FP64 eigensolve, coordinates); the build is once per dictionary",
         },
-        "cosine_error_vs_fp64": {
+        "cosine_error_vs_fp64": None if head_cos is None else {
             a.precision + ("_pca" if pca_search else "") + "+rescore": head_cos,
             "fp32": fp32_cos,
             "what": "|s - <q,d>_fp64| / (|q| |d|) for the reported top-1 of every pattern, <q,d>_fp64 the exact "
