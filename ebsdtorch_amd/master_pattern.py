@@ -17,7 +17,10 @@ from ._lib import check, require_device
 
 
 def space_group_to_laue(space_group: int) -> int:
-    """Space group number -> Laue group id (utils/symmetry_classes.py:4-30)."""
+    """Space group number -> Laue group id (utils/symmetry_classes.py:4-30),
+    the reference's table as it is: it maps 149-155 (point group 32) to Laue
+    group 6 and 194 (P6_3/mmc) to 10, where the International Tables have 7
+    and 9."""
     if space_group < 1 or space_group > 230:
         raise ValueError("The space group must be between 1 and 230 inclusive.")
     for bound, lid in ((206, 11), (193, 10), (176, 9), (167, 8), (155, 7), (142, 6), (88, 5), (74, 4), (15, 3),

@@ -203,3 +203,36 @@ def test_two_phase_driver_with_cpu_resident_patterns():
     q_sel = torch.where(want.bool()[:, None], exp.raw_indexing_results[1][0][:, 0].cpu(),
                         exp.raw_indexing_results[0][0][:, 0].cpu())
     assert torch.equal(exp.orientations.cpu(), q_sel)
+
+
+def test_di_on_a_master_pattern_read_from_an_emsoft_file():
+    """A master pattern read from an EMsoft-layout HDF5 file (energy-weighted
+    hemispheres, space group 225 -> m-3m): DI of noisy patterns simulated
+    from it, GPU against the oracle, top-1 equal except ties."""
+    import os
+
+    from ebsdtorch_amd.io import read_master_pattern
+
+    _say("DI on an EMsoft file's master pattern")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "emsoft_mp_m3m.h5")
+    mpo = read_master_pattern(path)
+    assert mpo.laue_group == 11
+    mp = mpo.master_pattern
+    rays = O.normalized_rays((30, 30), (0.5, 0.5, 0.6))
+    dico = O.sample_ori_fz_laue(11, 3000)
+    ori = synthetic.random_orientations(128, seed=41)
+    raw = O.interpolate(mp, O.qu_apply(ori[:, None], rays[None]), normalize_coords=False)
+    g = torch.Generator().manual_seed(42)
+    exp = (torch.poisson(300 * raw / raw.mean(1, keepdim=True), generator=g) / 300).contiguous()
+    ref_idx, _ = O.dictionary_index(mp, rays, exp, dico, top_k=1, dictionary_chunk_size=1000,
+                                    experiment_chunk_size=1000)
+    ep = E.ExperimentPatterns(exp.view(128, 30, 30))
+    E.dictionary_index_orientations(E.MasterPattern(mp.to(DEV), laue_group=mpo.laue_group),
+                                    E.EBSDGeometry((30, 30), proj_center=(0.5, 0.5, 0.6)), ep,
+                                    match_dtype=torch.float32, dictionary_orientations=[dico])
+    dict_pats = O.project_dictionary(mp, rays, dico, chunk=1000)
+    n_diff = _assert_top1_parity(ep.raw_indexing_indices[0][:, 0].cpu(), ref_idx[:, 0], exp, dict_pats)
+    _say(f"EMsoft-file DI: {n_diff} tie swaps of 128")
+    # most patterns find an orientation near their own (the 3,000-entry grid is coarse)
+    dis = torch.rad2deg(E.disori_angle_laue(ep.orientations.cpu().double(), ori.double(), 11))
+    assert float((dis < 8.0).float().mean()) > 0.9
