@@ -30,6 +30,8 @@ def load_reference():
     from ebsdtorch.s2_and_so3 import sampling, laue_fz_ori, quaternions, sphere, orientations, laue_generators
     from ebsdtorch.ebsd import geometry, master_pattern, experiment_pats
     from ebsdtorch.utils import knn, pca_covmat
+    from ebsdtorch.preprocessing import nlpar, clahe
+    from ebsdtorch.ebsd import refining
 
     src = open(os.path.join(REF, "ebsdtorch/ebsd/indexing.py")).read().splitlines()
     # the drivers end at line 476; the demo script starts at line 479
@@ -38,7 +40,7 @@ def load_reference():
     return dict(sampling=sampling, laue_fz_ori=laue_fz_ori, quaternions=quaternions, sphere=sphere,
                 orientations=orientations, laue_generators=laue_generators, geometry=geometry,
                 master_pattern=master_pattern, experiment_pats=experiment_pats, knn=knn,
-                pca_covmat=pca_covmat, indexing=ns)
+                pca_covmat=pca_covmat, indexing=ns, nlpar=nlpar, clahe=clahe, refining=refining)
 
 
 def unit_quats(rng, n):
@@ -186,6 +188,53 @@ def main():
     out["pca_dict"] = dico.numpy()
     out["pca_quats"] = ep.raw_indexing_results[0][0].numpy()
     out["pca_metrics"] = ep.raw_indexing_results[0][1].numpy()
+
+    # NLPAR (preprocessing/nlpar.py:20-192): a 7 x 9 scan of 6 x 5 patterns
+    # made of a few smooth clusters plus noise, k_rad 1 and 2
+    sy, sx = np.meshgrid(np.arange(7), np.arange(9), indexing="ij")
+    base = np.stack([np.sin(0.7 * (i + 1) * np.linspace(0, 3, 30)) for i in range(3)])
+    lab = ((sy // 3) + 2 * (sx // 5)) % 3
+    scan = base[lab].reshape(7, 9, 6, 5) + 0.3 * rng.standard_normal((7, 9, 6, 5))
+    scan = scan.astype(np.float32)
+    out["nlpar_in"] = scan
+    for k in (1, 2):
+        out[f"nlpar_k{k}"] = R["nlpar"].nlpar(torch.tensor(scan), k_rad=k).numpy()
+    out["nlpar_k1_bias"] = R["nlpar"].nlpar(torch.tensor(scan), k_rad=1, coeff=0.5,
+                                            center_logit_bias=0.3).numpy()
+
+    # CLAHE (preprocessing/clahe.py:18-130): images in [0, 1], shapes that
+    # need reflect padding, a clip that redistributes, an unclipped case
+    for tag, (b, hh, ww, clip, bins, grid) in {
+        "clahe_a": (3, 30, 26, 40.0, 64, (4, 4)),
+        "clahe_b": (2, 17, 23, 2.0, 16, (3, 2)),
+        "clahe_c": (2, 16, 16, 1000.0, 32, (2, 2)),
+    }.items():
+        img = rng.random((b, 1, hh, ww)).astype(np.float32) ** 2
+        img[0, 0, : hh // 2] *= 0.3  # a dark half: uneven tile histograms
+        out[f"{tag}_in"] = img
+        out[f"{tag}_out"] = R["clahe"].clahe_grayscale(torch.tensor(img), clip_limit=clip, n_bins=bins,
+                                                       grid_shape=grid).numpy()
+
+    # Orientation grid refinement (ebsd/refining.py:71-244): the DI problem
+    # above (33x66 MP, 8x8 detector, 16 noisy patterns) started from
+    # orientations 1.5 degrees off the truth, FP32 dots; unbinned and binned
+    geom8 = R["geometry"].EBSDGeometry((8, 8), proj_center=(0.5, 0.5, 0.6))
+    axis = unit_vecs(rng, 16).astype(np.float64)
+    ang = np.deg2rad(1.5)
+    dq = np.concatenate([np.full((16, 1), np.cos(ang / 2)), axis * np.sin(ang / 2)], 1)
+    start = R["quaternions"].qu_prod(torch.tensor(dq, dtype=torch.float32), torch.tensor(pats_q))
+    start = start * torch.where(start[:, :1] < 0, -1.0, 1.0)
+    out["refine_start"] = start.numpy()
+    for tag, binning in (("b1", 1), ("b2", 2)):
+        ep = R["experiment_pats"].ExperimentPatterns(torch.tensor(noisy))
+        ep.set_orientations(start.clone())
+        ep.phases = torch.zeros(16, dtype=torch.uint8)
+        MPr = R["master_pattern"].MasterPattern(torch.tensor(mp), laue_group=11)
+        dots = R["refining"].orientation_grid_refinement(
+            MPr, geom8, ep, grid_semi_edge_in_degrees=2.0, batch_size=4, virtual_binning=binning, n_iter=3,
+            kernel_radius_in_steps=1, shrink_factor=0.5, match_dtype=torch.float32)
+        out[f"refine_{tag}_ori"] = ep.get_orientations().numpy()
+        out[f"refine_{tag}_dots"] = dots[0].numpy()
 
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")
