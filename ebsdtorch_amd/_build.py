@@ -28,6 +28,7 @@ SOURCES = {
     "local.hip": [],
     "nlpar.hip": [],
     "preprocess.hip": [],
+    "fitting.hip": [],
 }
 
 COMMON_FLAGS = [

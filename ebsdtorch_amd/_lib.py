@@ -61,6 +61,7 @@ _SIGNATURES = {
     "ebsd_project_quads": (
         _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int,
                  _c_void_p, _c_int, _c_void_p, _c_void_p]),
+    "ebsd_cmif_maps": (_c_int, [_c_void_p, ctypes.c_longlong, _c_int, _c_void_p, _c_void_p]),
     "ebsd_view_keys": (
         _c_int, [_c_void_p, _c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _c_void_p, _c_void_p]),
     "ebsd_project_shifted": (

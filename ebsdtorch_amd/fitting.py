@@ -77,19 +77,50 @@ def periphery_rays(pixel_coords: torch.Tensor, rotation: torch.Tensor, translati
     return c / torch.norm(c, dim=-1, keepdim=True)
 
 
+def _levels(x: torch.Tensor, bins: int = 4) -> torch.Tensor:
+    """(N, 1, H, W) -> (N, bins, H, W) float32 indicators of trunc(bins x) == k
+    (utils/mutual_information.py:375-389, the uint8 cast of bins * x)."""
+    lv = (bins * x).to(torch.uint8)
+    return (lv == torch.arange(bins, dtype=torch.uint8, device=x.device)[None, :, None, None]).to(torch.float32)
+
+
+def _mi_maps(sim: torch.Tensor, src_fft: torch.Tensor, pshape, chunk: int = 128) -> torch.Tensor:
+    """cmif(images_dst=sim, images_src=patterns, src_ind=detector area, 4 + 4
+    bins) for dictionary patterns sim (D, 1, Hp, Wp): (D, B, Hp, Wp). The
+    25 cross-correlations of the indicator and level channels are FFT
+    products; ebsd_cmif_maps turns them into entropies per shift."""
+    d = sim.shape[0]
+    out = []
+    for s0 in range(0, d, chunk):
+        x = sim[s0:s0 + chunk]
+        dst = torch.cat((torch.ones_like(x), _levels(x)), 1)
+        dst_fft = torch.fft.rfft2(dst)  # (d, 5, Hp, Wf)
+        prod = dst_fft[:, None, :, None] * src_fft[None, :, None, :]  # (d, B, 5, 5, Hp, Wf)
+        corr = torch.fft.irfft2(prod)
+        del prod
+        out.append(torch.fft.fftshift(kernels.cmif_maps(corr), dim=(-2, -1)))
+        del corr
+    return torch.cat(out, 0)
+
+
 def spatial_fft_pc_fitter(mp: MasterPattern, patterns: torch.Tensor, pc_x_min: float = 0.0, pc_x_max: float = 1.0,
                           pc_y_min: float = 0.0, pc_y_max: float = 1.0, pc_z_min: float = 0.3,
                           pc_z_max: float = 0.7, pc_z_steps: int = 16, primary_tilt_deg: float = 70.0,
                           secondary_tilt_deg: float = 0.0, dictionary_resolution_deg: float = 1.0,
                           dictionary_indexing_batch_size: int = 1024, virtual_binning: int = 1,
                           metric: str = "dot", quats_dict: torch.Tensor = None):
-    """fitting.py:359-535 with metric "dot" on the GPU. patterns (B, 1, H, W)
-    float32 (modified in place, as the reference does). quats_dict: the
-    dictionary (default: sample_ori_fz_laue_angle at dictionary_resolution_deg
-    for the master pattern's Laue group). Returns (pattern centre (3,),
-    orientations (B, 4))."""
-    if metric != "dot":
-        raise NotImplementedError("only metric='dot' is implemented (the mutual-information metric is not)")
+    """fitting.py:359-535 on the GPU. patterns (B, 1, H, W) float32
+    (modified in place, as the reference does). quats_dict: the dictionary
+    (default: sample_ori_fz_laue_angle at dictionary_resolution_deg for the
+    master pattern's Laue group). metric "dot": FFT cross-correlation of the
+    zero-mean patterns; "mi": the mutual-information map of cmif with 4 + 4
+    bins (fitting.py:486-496): level-indicator images of each dictionary
+    pattern (all foreground) and each minmax-normalised experimental pattern
+    (its detector area as foreground), their 25 channel-pair
+    cross-correlations by FFT, and ebsd_cmif_maps for the entropies per
+    shift. Returns (pattern centre (3,), orientations (B, 4))."""
+    if metric not in ("dot", "mi"):
+        raise ValueError("Invalid metric.")
     if patterns.dim() != 4:
         raise ValueError("Pattern must be 4D (B, C, H, W).")
     B, C, H, W = patterns.shape
@@ -105,12 +136,19 @@ def spatial_fft_pc_fitter(mp: MasterPattern, patterns: torch.Tensor, pc_x_min: f
         patterns = torch.nn.functional.avg_pool2d(patterns, kernel_size=virtual_binning, stride=virtual_binning)
     patterns -= patterns.view(B, -1).min(dim=-1, keepdim=True).values.view(B, 1, 1, 1)
     patterns /= patterns.view(B, -1).max(dim=-1, keepdim=True).values.view(B, 1, 1, 1)
-    patterns -= patterns.view(B, -1).mean(dim=-1, keepdim=True).view(B, 1, 1, 1)
+    if metric == "dot":
+        patterns -= patterns.view(B, -1).mean(dim=-1, keepdim=True).view(B, 1, 1, 1)
 
     pix, padding, pshape = get_3D_pixel_coords_with_periphery((H, W), dev, pc_x_min, pc_x_max, pc_y_min, pc_y_max,
                                                               binning=virtual_binning)
     pad = torch.nn.functional.pad(patterns, padding, mode="constant", value=0.0).view(1, B, pshape[0], pshape[1])
-    pattern_fft = torch.fft.rfft2(pad).conj()
+    if metric == "dot":
+        pattern_fft = torch.fft.rfft2(pad).conj()
+    else:
+        ind = torch.nn.functional.pad(torch.ones((1, 1, hb, wb), device=dev), padding, mode="constant",
+                                      value=0.0).expand(B, 1, pshape[0], pshape[1])
+        src_fft = torch.fft.rfft2(torch.cat((ind, _levels(pad.view(B, 1, pshape[0], pshape[1]))), 1)).conj()
+        # (B, 5, Hp, Wf): conj(FFT) of the source indicator and level images
     if quats_dict is None:
         quats_dict = sample_ori_fz_laue_angle(mp.laue_group, dictionary_resolution_deg, dev)
     quats_dict = quats_dict.to(dev, torch.float32).contiguous()
@@ -129,8 +167,11 @@ def spatial_fft_pc_fitter(mp: MasterPattern, patterns: torch.Tensor, pc_x_min: f
             q = quats_dict[j:j + dictionary_indexing_batch_size]
             sim = kernels.project(mpb, rays, q, DTYPE_F32, zero_mean=False, quads=quads)
             sim = sim.data[:, :rays.shape[0]].reshape(q.shape[0], 1, pshape[0], pshape[1])
-            dots = torch.fft.fftshift(torch.fft.irfft2(torch.fft.rfft2(sim, dim=(-2, -1)) * pattern_fft,
-                                                       s=pshape), dim=(-2, -1))
+            if metric == "dot":
+                dots = torch.fft.fftshift(torch.fft.irfft2(torch.fft.rfft2(sim, dim=(-2, -1)) * pattern_fft,
+                                                           s=pshape), dim=(-2, -1))
+            else:
+                dots = _mi_maps(sim, src_fft, pshape)
             vals, idx = torch.max(dots, dim=0)
             best_dots[z_idx] = torch.maximum(best_dots[z_idx], vals)
             best_ori[z_idx] = torch.where(best_dots[z_idx] == vals, idx, best_ori[z_idx])

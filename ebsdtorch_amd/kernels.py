@@ -784,3 +784,20 @@ def rot6d_to_quat(z6: torch.Tensor) -> torch.Tensor:
                                            _lib.stream_ptr(z6.device))
     check(st, "ebsd_nn_rot6d_to_quat")
     return q
+
+
+def cmif_maps(corr: torch.Tensor) -> torch.Tensor:
+    """Mutual information per shift (ebsd_cmif_maps) from the 5 x 5
+    level-indicator cross-correlations corr (..., 5, 5, H, W) float32 on the
+    GPU: returns (..., H, W)."""
+    require_device(corr, "corr")
+    if corr.dtype != torch.float32 or corr.dim() < 4 or corr.shape[-4:-2] != (5, 5):
+        raise ValueError("corr must be float32 (..., 5, 5, H, W)")
+    corr = corr.contiguous()
+    lead = corr.shape[:-4]
+    h, w = corr.shape[-2:]
+    n_maps = int(torch.tensor(lead).prod().item()) if len(lead) else 1
+    out = torch.empty(lead + (h, w), dtype=torch.float32, device=corr.device)
+    check(_lib.load().ebsd_cmif_maps(corr.data_ptr(), n_maps, h * w, out.data_ptr(), _lib.stream_ptr(corr.device)),
+          "ebsd_cmif_maps")
+    return out

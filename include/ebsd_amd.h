@@ -472,6 +472,19 @@ int ebsd_clahe(const float* x, int n, int h, int w, int grid_h, int grid_w, int 
  */
 int ebsd_normalize_patterns(float* x, int n, int n_pix, int mode, void* stream);
 
+/*
+ * Mutual-information maps of the pattern-centre fitter's "mi" metric: for
+ * each of n_maps (dictionary pattern, experimental pattern) pairs, corr holds
+ * the 5 x 5 circular cross-correlations (n_maps, 5, 5, n_pix) f32 of their
+ * level-indicator images (channel 0 the foreground, 1-4 the value bins of
+ * trunc(4 v)); out (n_maps, n_pix) f32 receives H_dst + H_src - H_joint per
+ * shift, counts below 1 dropped, NaN / infinite / negative values set to 0.
+ * Replaces: utils/mutual_information.py:304-673 (cmif, bins 4 + 4,
+ * constant_marginal=False, norm_mode "none") after its FFT products, as
+ * ebsd/fitting.py:486-496 calls it.
+ */
+int ebsd_cmif_maps(const float* corr, long long n_maps, int n_pix, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

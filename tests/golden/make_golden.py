@@ -32,6 +32,7 @@ def load_reference():
     from ebsdtorch.utils import knn, pca_covmat
     from ebsdtorch.preprocessing import nlpar, clahe
     from ebsdtorch.ebsd import refining
+    from ebsdtorch.utils import mutual_information
 
     src = open(os.path.join(REF, "ebsdtorch/ebsd/indexing.py")).read().splitlines()
     # the drivers end at line 476; the demo script starts at line 479
@@ -40,7 +41,8 @@ def load_reference():
     return dict(sampling=sampling, laue_fz_ori=laue_fz_ori, quaternions=quaternions, sphere=sphere,
                 orientations=orientations, laue_generators=laue_generators, geometry=geometry,
                 master_pattern=master_pattern, experiment_pats=experiment_pats, knn=knn,
-                pca_covmat=pca_covmat, indexing=ns, nlpar=nlpar, clahe=clahe, refining=refining)
+                pca_covmat=pca_covmat, indexing=ns, nlpar=nlpar, clahe=clahe, refining=refining,
+                mutual_information=mutual_information)
 
 
 def unit_quats(rng, n):
@@ -235,6 +237,25 @@ def main():
             kernel_radius_in_steps=1, shrink_factor=0.5, match_dtype=torch.float32)
         out[f"refine_{tag}_ori"] = ep.get_orientations().numpy()
         out[f"refine_{tag}_dots"] = dots[0].numpy()
+
+    # cmif (utils/mutual_information.py:304-673) as the fitter's "mi" metric
+    # calls it (fitting.py:486-496): 4 + 4 bins, per-shift marginals;
+    # destination (dictionary) images on a full foreground, source
+    # (experimental) images zero-padded with their detector area as foreground
+    hh, ww = 12, 10
+    yy, xx = np.meshgrid(np.linspace(0, 1, hh), np.linspace(0, 1, ww), indexing="ij")
+    dst = np.stack([0.5 + 0.45 * np.sin(3 * xx + 2 * yy + ph) for ph in (0.0, 1.0, 2.5)])[:, None]
+    dst = np.clip(dst + 0.05 * rng.standard_normal(dst.shape), 0, 0.999).astype(np.float32)
+    src = np.zeros((2, 1, hh, ww), np.float32)
+    ind = np.zeros((2, 1, hh, ww), np.float32)
+    src[:, :, 2:10, 1:8] = np.stack([dst[0, 0, 2:10, 2:9], dst[2, 0, 3:11, 1:8]])[:, None]
+    src[:, :, 2:10, 1:8] += 0.05 * rng.standard_normal((2, 1, 8, 7)).astype(np.float32)
+    src = np.clip(src, 0, 0.999).astype(np.float32)
+    ind[:, :, 2:10, 1:8] = 1.0
+    out["cmif_dst"], out["cmif_src"], out["cmif_ind"] = dst, src, ind
+    out["cmif_out"] = R["mutual_information"].cmif(images_dst=torch.tensor(dst), images_src=torch.tensor(src),
+                                                   src_ind=torch.tensor(ind), bins_dst=4, bins_src=4,
+                                                   constant_marginal=False).numpy()
 
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")

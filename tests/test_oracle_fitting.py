@@ -1,6 +1,7 @@
 """Pin the pattern-centre fitter oracle (fitting.py:219-535) and the
 product's geometry helpers on the CPU."""
 
+import numpy as np
 import torch
 
 from oracle import di_oracle as O
@@ -60,3 +61,18 @@ def test_fitter_recovers_a_translated_pattern_centre():
     # geometry (geometry.py:31-92); restated as is
     assert abs(float(pc[0]) - (1 - truth[0])) < 1e-6 and abs(float(pc[1]) - (1 - truth[1])) < 1e-6
     assert torch.equal(q, quats[:3])
+
+
+def test_cmif_matches_reference_and_brute_force(golden):
+    """The oracle's cross mutual information map against the reference's own
+    cmif output (bins 4 + 4, per-shift marginals, a source foreground mask)
+    and against direct counting of the pixel pairs at every shift."""
+    from oracle import fitting_oracle as FO
+
+    dst, src, ind = (torch.tensor(golden[k]) for k in ("cmif_dst", "cmif_src", "cmif_ind"))
+    got = FO.cmif(dst, src, ind)
+    np.testing.assert_allclose(got.numpy(), golden["cmif_out"], rtol=1e-5, atol=2e-6)
+    brute, ones = FO.cmif_brute(dst[:2], src, ind)
+    keep = ~ones
+    assert int(keep.sum()) > 100
+    np.testing.assert_allclose(got[:2].double()[keep].numpy(), brute[keep].numpy(), rtol=1e-4, atol=1e-5)
