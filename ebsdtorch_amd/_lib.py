@@ -15,7 +15,7 @@ import torch
 
 from ._build import LIB_PATH
 
-EBSD_ABI_VERSION = 2
+EBSD_ABI_VERSION = 3
 
 EBSD_OK = 0
 EBSD_ERR_NULL = -1
@@ -25,6 +25,8 @@ EBSD_ERR_TOPK = -4
 EBSD_ERR_WORKSPACE = -5
 EBSD_ERR_ALIGN = -6
 EBSD_ERR_SCRATCH = -7
+EBSD_ERR_INTERNAL = -8
+EBSD_SCHEDULE_INFO = 11
 EBSD_NOALLOC_SCRATCH_BYTES = 64 << 20
 EBSD_MAX_TOPK = 16
 
@@ -140,6 +142,11 @@ _SIGNATURES = {
         _c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, ctypes.c_longlong, ctypes.c_float, ctypes.c_float,
                  ctypes.c_float, ctypes.c_float, _c_int, ctypes.c_float, _c_void_p, _c_void_p]),
     "ebsd_sim_topk_workspace": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_c_size_t)]),
+    "ebsd_sim_schedule": (
+        _c_int,
+        [_c_int, _c_int, _c_int, _c_int, _c_int, _c_void_p, _c_void_p, ctypes.c_longlong, _c_void_p,
+         ctypes.c_longlong],
+    ),
     "ebsd_sim_topk": (
         _c_int,
         [_c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,

@@ -14,6 +14,7 @@ extern "C" const char* ebsd_status_string(int status) {
     case EBSD_ERR_WORKSPACE: return "workspace missing or too small";
     case EBSD_ERR_ALIGN: return "pointer not 16-byte aligned";
     case EBSD_ERR_SCRATCH: return "sizes exceed the static scratch buffer of the allocation-free call";
+    case EBSD_ERR_INTERNAL: return "the library's work plan is inconsistent (library bug)";
     default: break;
   }
   if (status > 0) return hipGetErrorString(static_cast<hipError_t>(status));

@@ -16,7 +16,7 @@
 // Numerics: rotation, Lambert arithmetic and the bilinear blend round step by
 // step like the reference's float32 torch expressions (contraction is off
 // inside them); the atan of the Lambert map is this library's own
-// polynomial (<= 1.44 ulp) evaluated on a reciprocal-based ratio.
+// polynomial (<= 1.415 ulp) evaluated on a reciprocal-based ratio.
 
 #include "common.h"
 #include "lambert.h"

@@ -53,6 +53,43 @@ struct SimSched {
   int tiles_per_part;  // dictionary tiles per part (n_parts = B * R parts)
 };
 
+// What one persistent workgroup of either similarity kernel does: for query
+// groups g = g_first, g_first + g_step, ... < n_qgroups it takes query tile
+// g * A + a (when that tile exists) against dictionary tiles
+// [t_begin, t_end) and writes partial list `part` of the tile's queries.
+// Workgroups are dispatched round-robin over the 8 XCDs, so block % 8 is the
+// XCD and block / 8 the slot (a, b): the B workgroups of a query tile share
+// its stages in the XCD's L2, and the A workgroups of a part stream the same
+// dictionary tiles in the same order. With fewer than 8 query groups the
+// XCDs split the dictionary into R super-parts instead.
+struct SlotWork {
+  bool idle;
+  int a, part, g_first, g_step, t_begin, t_end;
+};
+
+__host__ __device__ inline SlotWork slot_work(const SimSched& sc, int block, int n_tiles) {
+  SlotWork w;
+  const int xcd = block & 7;
+  const int slot = block >> 3;
+  w.a = slot / sc.B;
+  const int b = slot - w.a * sc.B;
+  int super;
+  if (sc.R == 1) {
+    w.g_first = xcd;
+    w.g_step = 8;
+    super = 0;
+  } else {
+    w.g_first = xcd % sc.n_qgroups;
+    w.g_step = sc.n_qgroups;
+    super = xcd / sc.n_qgroups;
+  }
+  w.idle = super >= sc.R;
+  w.part = super * sc.B + b;
+  w.t_begin = w.part * sc.tiles_per_part;
+  w.t_end = w.t_begin + sc.tiles_per_part < n_tiles ? w.t_begin + sc.tiles_per_part : n_tiles;
+  return w;
+}
+
 template <int DT, int K>
 __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
     const unsigned char* __restrict__ Q, const float* __restrict__ q_scale, int M,
@@ -62,34 +99,15 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  // Workgroups are dispatched round-robin over the 8 XCDs, so blockIdx % 8 is
-  // this workgroup's XCD and blockIdx / 8 its slot among that XCD's 64.
-  // Slot (a, b) takes query tile a of the XCD's current group of A tiles and
-  // dictionary part b: the B workgroups of a query tile share its stages in
-  // the XCD's L2, and the A workgroups of a part stream the same dictionary
-  // tiles in the same order.
-  const int xcd = blockIdx.x & 7;
-  const int slot = blockIdx.x >> 3;
-  const int a = slot / sc.B;
-  const int b = slot - a * sc.B;
-  int g_first, g_step, super;
-  if (sc.R == 1) {
-    g_first = xcd;
-    g_step = 8;
-    super = 0;
-  } else {
-    g_first = xcd % sc.n_qgroups;
-    g_step = sc.n_qgroups;
-    super = xcd / sc.n_qgroups;
-    if (super >= sc.R) return;
-  }
-  const int part = super * sc.B + b;
-  const int n_tiles = (N + TILE - 1) / TILE;
-  const int t_begin = part * sc.tiles_per_part;
-  const int t_end = min(t_begin + sc.tiles_per_part, n_tiles);
+  // 64 slots per XCD (two workgroups per CU); the mapping is slot_work's
+  const SlotWork wk = slot_work(sc, blockIdx.x, (N + TILE - 1) / TILE);
+  if (wk.idle) return;
+  const int part = wk.part;
+  const int t_begin = wk.t_begin;
+  const int t_end = wk.t_end;
 
-  for (int g = g_first; g < sc.n_qgroups; g += g_step) {
-  const int qt = g * sc.A + a;
+  for (int g = wk.g_first; g < sc.n_qgroups; g += wk.g_step) {
+  const int qt = g * sc.A + wk.a;
   if (qt >= sc.n_qtiles) continue;  // uniform over the workgroup
   const int q0 = qt * TILE;
   const int split = part;
@@ -518,28 +536,12 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
   const int wave = tid >> 6;
   const int wd = wave & 1, wq = wave >> 1;
   const int r32 = lane & 31, h = lane >> 5;
-  // Workgroups are dispatched round-robin over the XCDs: blockIdx % 8 is the
-  // XCD, blockIdx / 8 the slot (a, b): query tile a of the XCD's group of A,
-  // dictionary part b (see the FP32 kernel for the grouping).
-  const int xcd = blockIdx.x & 7;
-  const int slot = blockIdx.x >> 3;
-  const int a = slot / sc.B;
-  const int b = slot - a * sc.B;
-  int g_first, g_step, super;
-  if (sc.R == 1) {
-    g_first = xcd;
-    g_step = 8;
-    super = 0;
-  } else {
-    g_first = xcd % sc.n_qgroups;
-    g_step = sc.n_qgroups;
-    super = xcd / sc.n_qgroups;
-    if (super >= sc.R) return;
-  }
-  const int part = super * sc.B + b;
-  const int n_tiles = (N + BT_D - 1) / BT_D;
-  const int t_begin = part * sc.tiles_per_part;
-  const int t_end = min(t_begin + sc.tiles_per_part, n_tiles);
+  // 32 slots per XCD (one workgroup per CU); the mapping is slot_work's
+  const SlotWork wk = slot_work(sc, blockIdx.x, (N + BT_D - 1) / BT_D);
+  if (wk.idle) return;
+  const int part = wk.part;
+  const int t_begin = wk.t_begin;
+  const int t_end = wk.t_end;
   const int nk = ld_bytes / BT_KB;
   const int G = max(t_end - t_begin, 0) * nk;  // stages per work unit
   // direct-load lane mapping: load L writes rows 16 L .. 16 L + 15 of the
@@ -553,8 +555,8 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
   float* lthr = reinterpret_cast<float*>(smem + BT_THR_OFF);
   float* scratch = reinterpret_cast<float*>(smem + BT_SCR_OFF) + wave * 64 * BT_SCR_ROW;
 
-  for (int grp = g_first; grp < sc.n_qgroups; grp += g_step) {
-    const int qt = grp * sc.A + a;
+  for (int grp = wk.g_first; grp < sc.n_qgroups; grp += wk.g_step) {
+    const int qt = grp * sc.A + wk.a;
     if (qt >= sc.n_qtiles) continue;  // uniform over the workgroup
     // (a part with no tiles, G == 0, still writes its empty lists)
     const int q0 = qt * BT_Q;
@@ -913,6 +915,45 @@ extern "C" int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t
   const size_t b2 = workspace_bytes(plan_sim(m, n, topk, dtype, kBtShortRowBytes), m);
   const size_t b3 = workspace_bytes(plan_sim(m, n, topk, dtype, kBtShortRowBytes + BT_KB), m);
   *bytes = std::max(b1, std::max(b2, b3));
+  return EBSD_OK;
+}
+
+extern "C" int ebsd_sim_schedule(int m, int n, int topk, int dtype, int ld, int* info, int* tile_visits,
+                                 long long visits_len, int* list_writes, long long writes_len) {
+  if (!info) return EBSD_ERR_NULL;
+  if (m < 1 || n < 1 || ld < 1) return EBSD_ERR_SHAPE;
+  if (topk < 1 || topk > EBSD_MAX_TOPK) return EBSD_ERR_TOPK;
+  const int eb = elem_bytes(dtype);
+  if (eb == 0) return EBSD_ERR_DTYPE;
+  if (ld % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
+  const SimPlan p = plan_sim(m, n, topk, dtype, ld * eb);
+  const int q_rows = p.bt ? BT_Q : TILE;
+  const int d_rows = p.bt ? BT_D : TILE;
+  const int n_dtiles = (n + d_rows - 1) / d_rows;
+  const int vals[EBSD_SCHEDULE_INFO] = {p.bt ? 1 : 0, p.grid, p.n_parts, p.k_list, q_rows,
+                                        d_rows, p.sc.n_qtiles, n_dtiles, p.sc.A, p.sc.B, p.sc.R};
+  for (int i = 0; i < EBSD_SCHEDULE_INFO; ++i) info[i] = vals[i];
+  const long long nv = (long long)p.sc.n_qtiles * n_dtiles;
+  const long long nw = (long long)p.sc.n_qtiles * p.n_parts;
+  if (tile_visits && visits_len < nv) return EBSD_ERR_WORKSPACE;
+  if (list_writes && writes_len < nw) return EBSD_ERR_WORKSPACE;
+  if (tile_visits) std::fill(tile_visits, tile_visits + nv, 0);
+  if (list_writes) std::fill(list_writes, list_writes + nw, 0);
+  // the kernels' own loop over query groups and dictionary tiles, per block
+  for (int block = 0; block < p.grid; ++block) {
+    const SlotWork w = slot_work(p.sc, block, n_dtiles);
+    if (w.idle) continue;
+    for (int g = w.g_first; g < p.sc.n_qgroups; g += w.g_step) {
+      const int qt = g * p.sc.A + w.a;
+      if (qt >= p.sc.n_qtiles) continue;
+      if (w.part < 0 || w.part >= p.n_parts) return EBSD_ERR_INTERNAL;
+      if (list_writes) ++list_writes[(long long)qt * p.n_parts + w.part];
+      for (int t = w.t_begin; t < w.t_end; ++t) {
+        if (t < 0 || t >= n_dtiles) return EBSD_ERR_INTERNAL;
+        if (tile_visits) ++tile_visits[(long long)qt * n_dtiles + t];
+      }
+    }
+  }
   return EBSD_OK;
 }
 

@@ -267,6 +267,31 @@ def sim_topk(queries: PatternMatrix, dictionary: PatternMatrix, topk: int, top_v
     check(st, "ebsd_sim_topk")
 
 
+_SCHEDULE_KEYS = ("bt", "grid", "n_parts", "k_list", "q_rows", "d_rows", "n_qtiles", "n_dtiles", "A", "B", "R")
+
+
+def sim_schedule(m: int, n: int, topk: int, dtype: int, ld: int, counts: bool = True) -> dict:
+    """The work plan of sim_topk for m queries x n dictionary rows of width
+    ld (ebsd_sim_schedule), walked on the host with the kernels' own block ->
+    work mapping; no GPU is needed. With counts, "tile_visits" (query tiles
+    x dictionary tiles) and "list_writes" (query tiles x partial lists) are
+    int32 arrays of how often each tile pair is scored and each partial list
+    written."""
+    import numpy as np
+
+    lib = _lib.load()
+    info = (ctypes.c_int * _lib.EBSD_SCHEDULE_INFO)()
+    check(lib.ebsd_sim_schedule(m, n, topk, dtype, ld, info, None, 0, None, 0), "ebsd_sim_schedule")
+    out = dict(zip(_SCHEDULE_KEYS, list(info)))
+    if counts:
+        visits = np.empty((out["n_qtiles"], out["n_dtiles"]), dtype=np.int32)
+        writes = np.empty((out["n_qtiles"], out["n_parts"]), dtype=np.int32)
+        check(lib.ebsd_sim_schedule(m, n, topk, dtype, ld, info, visits.ctypes.data, visits.size,
+                                    writes.ctypes.data, writes.size), "ebsd_sim_schedule")
+        out["tile_visits"], out["list_writes"] = visits, writes
+    return out
+
+
 def sim_topk_noalloc(queries: PatternMatrix, dictionary: PatternMatrix, topk: int, top_val: torch.Tensor,
                      top_idx: torch.Tensor, index_base: int = 0, merge: bool = True) -> None:
     """sim_topk through the allocation-free entry point

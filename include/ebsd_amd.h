@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EBSD_ABI_VERSION 2
+#define EBSD_ABI_VERSION 3
 
 /* status codes */
 #define EBSD_OK 0
@@ -37,6 +37,7 @@ extern "C" {
 #define EBSD_ERR_WORKSPACE (-5)   /* workspace missing or too small         */
 #define EBSD_ERR_ALIGN (-6)       /* pointer not 16-byte aligned            */
 #define EBSD_ERR_SCRATCH (-7)     /* the sizes need more than the static scratch of an allocation-free call */
+#define EBSD_ERR_INTERNAL (-8)    /* the library's own work plan is inconsistent (a library bug) */
 
 #define EBSD_MAX_TOPK 16
 
@@ -167,6 +168,22 @@ int ebsd_prep_patterns_pair(const float* pats, int n, int ld_in, const int* pix_
 
 /* Bytes of scratch that ebsd_sim_topk needs for these sizes. */
 int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t* bytes);
+
+/*
+ * Work schedule of ebsd_sim_topk for these sizes, walked on the host with the
+ * kernels' own block -> work mapping (no device work). info receives
+ * EBSD_SCHEDULE_INFO ints: kernel (0 = 128 x 128, 1 = 192 x 256), grid
+ * (workgroups), partial lists per query, list length, query rows per tile,
+ * dictionary rows per tile, query tiles, dictionary tiles, and the grouping
+ * A, B, R. If given, tile_visits (query tiles x dictionary tiles, row-major)
+ * receives how many workgroups score each tile pair, and list_writes (query
+ * tiles x partial lists) how many write each partial list; a correct plan
+ * has every entry equal to 1. EBSD_ERR_WORKSPACE when an array is too short.
+ * No counterpart in the reference: it lets a host check the schedule.
+ */
+#define EBSD_SCHEDULE_INFO 11
+int ebsd_sim_schedule(int m, int n, int topk, int dtype, int ld, int* info, int* tile_visits,
+                      long long visits_len, int* list_writes, long long writes_len);
 
 /*
  * Similarity + streaming top-k: scores S = Q * Dict^T (angular metric, FP32
