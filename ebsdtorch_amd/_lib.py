@@ -27,6 +27,8 @@ EBSD_ERR_ALIGN = -6
 EBSD_ERR_SCRATCH = -7
 EBSD_ERR_INTERNAL = -8
 EBSD_SCHEDULE_INFO = 11
+SAMPLE_MODES = {"bilinear": 0, "nearest": 1, "bicubic": 2}
+PADDING_MODES = {"zeros": 0, "border": 1, "reflection": 2}
 EBSD_NOALLOC_SCRATCH_BYTES = 64 << 20
 EBSD_MAX_TOPK = 16
 
@@ -76,6 +78,9 @@ _SIGNATURES = {
                  ctypes.c_ulonglong, ctypes.c_ulonglong, _c_void_p]),
     "ebsd_sample_dirs": (
         _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, ctypes.c_longlong, _c_void_p, _c_int, _c_void_p]),
+    "ebsd_sample_dirs_mode": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, ctypes.c_longlong, _c_void_p, _c_int, _c_int, _c_int,
+                 _c_int, _c_void_p]),
     "ebsd_prep_patterns": (
         _c_int,
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int,

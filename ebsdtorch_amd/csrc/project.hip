@@ -302,6 +302,117 @@ __global__ __launch_bounds__(256) void sample_dirs_kernel(const float* __restric
   out[i] = sample_border(mp, H, W, gx, gy);
 }
 
+// grid_sample's other modes (torch.nn.functional.grid_sample, as
+// MasterPattern.interpolate passes them through, master_pattern.py:197-206),
+// with the float32 operation sequence of torch's CPU kernel: source index
+// (g + 1) * size / 2 - 0.5 (align_corners=False) or (g + 1) * (size - 1) / 2;
+// border clips the coordinate to [0, size - 1]; reflection folds it about
+// [-0.5, size - 0.5] (or [0, size - 1]) and then clips; zeros reads taps
+// outside the image as 0. Bilinear and nearest pad the coordinate, bicubic
+// pads each of its 16 integer taps (Keys' cubic convolution, A = -0.75).
+template <int PAD, bool ALIGN>
+struct SampleAxis {
+  int size;
+  float max_val, scale, low, twice_span;
+  __device__ explicit SampleAxis(int n)
+      : size(n),
+        max_val((float)(n - 1)),
+        scale(ALIGN ? (float)(n - 1) / 2.0f : (float)n / 2.0f),
+        low(ALIGN ? 0.0f : -0.5f),
+        twice_span(ALIGN ? (float)(n - 1) * 2.0f : (float)n * 2.0f) {}
+  __device__ float unnormalize(float g) const {
+#pragma clang fp contract(off)
+    return ALIGN ? (g + 1.0f) * scale : (g + 1.0f) * scale - 0.5f;
+  }
+  __device__ float clip(float v) const { return fminf(max_val, fmaxf(0.0f, v)); }
+  __device__ float reflect(float v) const {
+#pragma clang fp contract(off)
+    if (twice_span == 0.0f) return 0.0f;
+    const float a = fabsf(v - low);
+    const float flips = truncf(a / twice_span);
+    const float extra = a - flips * twice_span;
+    return fminf(extra, twice_span - extra) + low;
+  }
+  // padding applied to a continuous (or integer-valued) coordinate
+  __device__ float pad(float v) const {
+    if constexpr (PAD == EBSD_PAD_BORDER) return clip(v);
+    else if constexpr (PAD == EBSD_PAD_REFLECTION) return clip(reflect(v));
+    else return v;
+  }
+  __device__ bool inside(int i) const { return i > -1 && i < size; }
+};
+
+template <int PAD, bool ALIGN>
+__device__ __forceinline__ float sample_tap(const float* __restrict__ img, const SampleAxis<PAD, ALIGN>& ax,
+                                            const SampleAxis<PAD, ALIGN>& ay, int xi, int yi) {
+  if (!ax.inside(xi) || !ay.inside(yi)) return 0.0f;
+  return __ldg(img + (yi * ax.size + xi));
+}
+
+__device__ __forceinline__ void cubic_coefficients(float t, float (&c)[4]) {
+#pragma clang fp contract(off)
+  const float A = -0.75f;
+  float x = t + 1.0f;
+  c[0] = ((A * x - 5.0f * A) * x + 8.0f * A) * x - 4.0f * A;
+  x = t;
+  c[1] = ((A + 2.0f) * x - (A + 3.0f)) * x * x + 1.0f;
+  x = 1.0f - t;
+  c[2] = ((A + 2.0f) * x - (A + 3.0f)) * x * x + 1.0f;
+  x = 2.0f - t;
+  c[3] = ((A * x - 5.0f * A) * x + 8.0f * A) * x - 4.0f * A;
+}
+
+template <int MODE, int PAD, bool ALIGN>
+__device__ float sample_mode(const float* __restrict__ img, int H, int W, float gx, float gy) {
+#pragma clang fp contract(off)
+  const SampleAxis<PAD, ALIGN> ax(W), ay(H);
+  if constexpr (MODE == EBSD_SAMPLE_BICUBIC) {
+    const float x = ax.unnormalize(gx), y = ay.unnormalize(gy);
+    const float fx = floorf(x), fy = floorf(y);
+    float cx[4], cy[4];
+    cubic_coefficients(x - fx, cx);
+    cubic_coefficients(y - fy, cy);
+    int xi[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) xi[i] = (int)ax.pad(fx + (float)(i - 1));
+    float out = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int yi = (int)ay.pad(fy + (float)(j - 1));
+      const float row = ((cx[0] * sample_tap(img, ax, ay, xi[0], yi) + cx[1] * sample_tap(img, ax, ay, xi[1], yi)) +
+                         cx[2] * sample_tap(img, ax, ay, xi[2], yi)) +
+                        cx[3] * sample_tap(img, ax, ay, xi[3], yi);
+      out = j == 0 ? cy[0] * row : out + cy[j] * row;
+    }
+    return out;
+  } else {
+    const float x = ax.pad(ax.unnormalize(gx)), y = ay.pad(ay.unnormalize(gy));
+    if constexpr (MODE == EBSD_SAMPLE_NEAREST) {
+      return sample_tap(img, ax, ay, (int)rintf(x), (int)rintf(y));
+    } else {
+      const float x_w = floorf(x), y_n = floorf(y);
+      const float w = x - x_w, e = 1.0f - w, n = y - y_n, s = 1.0f - n;
+      const int ix = (int)x_w, iy = (int)y_n;
+      return ((sample_tap(img, ax, ay, ix, iy) * (s * e) + sample_tap(img, ax, ay, ix + 1, iy) * (s * w)) +
+              sample_tap(img, ax, ay, ix, iy + 1) * (n * e)) +
+             sample_tap(img, ax, ay, ix + 1, iy + 1) * (n * w);
+    }
+  }
+}
+
+template <int MODE, int PAD, bool ALIGN>
+__global__ __launch_bounds__(256) void sample_dirs_mode_kernel(const float* __restrict__ mp, int H, int W,
+                                                               const float* __restrict__ dirs, int64_t n,
+                                                               float* __restrict__ out, int renorm) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+  if (renorm) renormalise(x, y, z);
+  float gx, gy;
+  lambert_sbs(x, y, z, gx, gy);
+  out[i] = sample_mode<MODE, PAD, ALIGN>(mp, H, W, gx, gy);
+}
+
 template <typename OutT>
 int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, const float* quats,
                    int n_ori, void* out, int ld, float* out_scale, int flags, hipStream_t st,
@@ -525,6 +636,56 @@ extern "C" int ebsd_sample_dirs(const float* mp, int mp_rows, int mp_cols, const
   if (blocks > 0x7fffffffLL) return EBSD_ERR_SHAPE;
   hipLaunchKernelGGL(sample_dirs_kernel, dim3((unsigned)blocks), dim3(256), 0,
                      static_cast<hipStream_t>(stream), mp, mp_rows, mp_cols, dirs, (int64_t)n, out, renorm);
+  EBSD_LAUNCH_CHECK();
+  return EBSD_OK;
+}
+
+template <int MODE, int PAD>
+static void launch_sample_mode(bool align, dim3 grid, hipStream_t st, const float* mp, int H, int W,
+                               const float* dirs, int64_t n, float* out, int renorm) {
+  if (align)
+    hipLaunchKernelGGL((sample_dirs_mode_kernel<MODE, PAD, true>), grid, dim3(256), 0, st, mp, H, W, dirs, n, out,
+                       renorm);
+  else
+    hipLaunchKernelGGL((sample_dirs_mode_kernel<MODE, PAD, false>), grid, dim3(256), 0, st, mp, H, W, dirs, n, out,
+                       renorm);
+}
+
+template <int MODE>
+static void launch_sample_pad(int padding, bool align, dim3 grid, hipStream_t st, const float* mp, int H, int W,
+                              const float* dirs, int64_t n, float* out, int renorm) {
+  switch (padding) {
+    case EBSD_PAD_ZEROS: launch_sample_mode<MODE, EBSD_PAD_ZEROS>(align, grid, st, mp, H, W, dirs, n, out, renorm); break;
+    case EBSD_PAD_BORDER: launch_sample_mode<MODE, EBSD_PAD_BORDER>(align, grid, st, mp, H, W, dirs, n, out, renorm); break;
+    default: launch_sample_mode<MODE, EBSD_PAD_REFLECTION>(align, grid, st, mp, H, W, dirs, n, out, renorm); break;
+  }
+}
+
+extern "C" int ebsd_sample_dirs_mode(const float* mp, int mp_rows, int mp_cols, const float* dirs, long long n,
+                                     float* out, int renorm, int mode, int padding, int align_corners,
+                                     void* stream) {
+  if (!mp || !dirs || !out) return EBSD_ERR_NULL;
+  if (mp_rows < 1 || mp_cols < 1 || n < 0) return EBSD_ERR_SHAPE;
+  if ((int64_t)mp_rows * mp_cols > (int64_t)INT_MAX) return EBSD_ERR_SHAPE;
+  if (mode < EBSD_SAMPLE_BILINEAR || mode > EBSD_SAMPLE_BICUBIC) return EBSD_ERR_SHAPE;
+  if (padding < EBSD_PAD_ZEROS || padding > EBSD_PAD_REFLECTION) return EBSD_ERR_SHAPE;
+  if (n == 0) return EBSD_OK;
+  const int64_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffffLL) return EBSD_ERR_SHAPE;
+  const dim3 grid((unsigned)blocks);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool align = align_corners != 0;
+  switch (mode) {
+    case EBSD_SAMPLE_BILINEAR:
+      launch_sample_pad<EBSD_SAMPLE_BILINEAR>(padding, align, grid, st, mp, mp_rows, mp_cols, dirs, n, out, renorm);
+      break;
+    case EBSD_SAMPLE_NEAREST:
+      launch_sample_pad<EBSD_SAMPLE_NEAREST>(padding, align, grid, st, mp, mp_rows, mp_cols, dirs, n, out, renorm);
+      break;
+    default:
+      launch_sample_pad<EBSD_SAMPLE_BICUBIC>(padding, align, grid, st, mp, mp_rows, mp_cols, dirs, n, out, renorm);
+      break;
+  }
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;
 }

@@ -103,16 +103,32 @@ class MasterPattern(torch.nn.Module):
     def interpolate(self, coords_3D: torch.Tensor, mode: str = "bilinear", padding_mode: str = "border",
                     align_corners: bool = False, normalize_coords: bool = True,
                     virtual_binning: int = 1) -> torch.Tensor:
-        """Master-pattern values at sphere points (..., 3) -> (...)."""
-        if mode != "bilinear" or padding_mode != "border" or align_corners:
-            raise NotImplementedError("the HIP lookup implements bilinear / border / align_corners=False, "
-                                      "the dictionary-indexing configuration")
+        """Master-pattern values at sphere points (..., 3) -> (...)
+        (master_pattern.py:141-209): the side-by-side Lambert coordinates of
+        each point sampled as grid_sample samples with this mode ("bilinear",
+        "nearest", "bicubic"), padding ("zeros", "border", "reflection") and
+        corner alignment. The defaults are the dictionary-indexing lookup
+        (ebsd_sample_dirs); the other settings run ebsd_sample_dirs_mode."""
+        if mode not in _lib.SAMPLE_MODES:
+            raise ValueError(f"nn.functional.grid_sample(): expected mode to be 'bilinear', 'nearest' or "
+                             f"'bicubic', but got: '{mode}'")
+        if padding_mode not in _lib.PADDING_MODES:
+            raise ValueError(f"nn.functional.grid_sample(): expected padding_mode to be 'zeros', 'border', or "
+                             f"'reflection', but got: '{padding_mode}'")
         require_device(coords_3D, "coords_3D")
         mp = self.pattern_for_binning(virtual_binning)
         dirs = coords_3D.reshape(-1, 3).float().contiguous()
         out = torch.empty(dirs.shape[0], dtype=torch.float32, device=dirs.device)
-        st = _lib.load().ebsd_sample_dirs(mp.data_ptr(), mp.shape[0], mp.shape[1], dirs.data_ptr(), dirs.shape[0],
-                                          out.data_ptr(), 1 if normalize_coords else 0,
-                                          _lib.stream_ptr(dirs.device))
-        check(st, "ebsd_sample_dirs")
+        lib = _lib.load()
+        renorm = 1 if normalize_coords else 0
+        stream = _lib.stream_ptr(dirs.device)
+        if mode == "bilinear" and padding_mode == "border" and not align_corners:
+            st = lib.ebsd_sample_dirs(mp.data_ptr(), mp.shape[0], mp.shape[1], dirs.data_ptr(), dirs.shape[0],
+                                      out.data_ptr(), renorm, stream)
+            check(st, "ebsd_sample_dirs")
+        else:
+            st = lib.ebsd_sample_dirs_mode(mp.data_ptr(), mp.shape[0], mp.shape[1], dirs.data_ptr(),
+                                           dirs.shape[0], out.data_ptr(), renorm, _lib.SAMPLE_MODES[mode],
+                                           _lib.PADDING_MODES[padding_mode], 1 if align_corners else 0, stream)
+            check(st, "ebsd_sample_dirs_mode")
         return out.view(coords_3D.shape[:-1])

@@ -128,6 +128,14 @@ int ebsd_project_shifted(const float* mp, int mp_rows, int mp_cols,
                          int out_dtype, int ld_out, float* out_scale, int flags,
                          void* stream);
 
+/* grid_sample modes and paddings of ebsd_sample_dirs_mode */
+#define EBSD_SAMPLE_BILINEAR 0
+#define EBSD_SAMPLE_NEAREST 1
+#define EBSD_SAMPLE_BICUBIC 2
+#define EBSD_PAD_ZEROS 0
+#define EBSD_PAD_BORDER 1
+#define EBSD_PAD_REFLECTION 2
+
 /*
  * Master-pattern lookup at arbitrary directions: out[i] = bilinear sample
  * (border, align_corners=False) of the side-by-side Lambert master pattern at
@@ -138,6 +146,20 @@ int ebsd_project_shifted(const float* mp, int mp_rows, int mp_cols,
  */
 int ebsd_sample_dirs(const float* mp, int mp_rows, int mp_cols, const float* dirs,
                      long long n, float* out, int renorm, void* stream);
+
+/*
+ * Master-pattern lookup at arbitrary directions in any grid_sample mode:
+ * mode EBSD_SAMPLE_BILINEAR / NEAREST / BICUBIC, padding EBSD_PAD_ZEROS /
+ * BORDER / REFLECTION, align_corners 0 or 1, each as torch's grid_sample
+ * defines it (nearest rounds half to even; bicubic is Keys' cubic
+ * convolution with A = -0.75, padding applied to each of its 16 taps).
+ *   dirs (n, 3) f32; out (n,) f32
+ * Replaces: ebsd/master_pattern.py:141-209 (MasterPattern.interpolate with
+ * the other modes; refining.py:203-209 uses bilinear and :338-344 bicubic,
+ * both with align_corners=True).
+ */
+int ebsd_sample_dirs_mode(const float* mp, int mp_rows, int mp_cols, const float* dirs, long long n,
+                          float* out, int renorm, int mode, int padding, int align_corners, void* stream);
 
 /*
  * Experimental-pattern preparation: gather pixels (pix_idx may be NULL for

@@ -448,16 +448,19 @@ def inv_rosca_lambert(pts: torch.Tensor) -> torch.Tensor:
     return torch.stack((x_out, y_out, z_out), dim=-1)
 
 
-def interpolate(mp: torch.Tensor, coords_3d: torch.Tensor, normalize_coords: bool = True) -> torch.Tensor:
-    """MasterPattern.interpolate, bilinear / border / align_corners=False
-    (master_pattern.py:141-209). The near-square reshape of :186-195 only
-    changes the grid's shape, not the values, so a (1, 1, n, 2) grid is used."""
+def interpolate(mp: torch.Tensor, coords_3d: torch.Tensor, normalize_coords: bool = True,
+                mode: str = "bilinear", padding_mode: str = "border", align_corners: bool = False) -> torch.Tensor:
+    """MasterPattern.interpolate (master_pattern.py:141-209): side-by-side
+    Lambert coordinates sampled by grid_sample with the given mode, padding
+    and corner alignment (defaults: the dictionary-indexing lookup). The
+    near-square reshape of :186-195 only changes the grid's shape, not the
+    values, so a (1, 1, n, 2) grid is used."""
     if normalize_coords:
         coords_3d = coords_3d / torch.norm(coords_3d, dim=-1, keepdim=True)
     g = rosca_lambert_side_by_side(coords_3d)
     n = g.numel() // 2
-    out = F.grid_sample(mp[None, None], g.reshape(1, 1, n, 2), mode="bilinear",
-                        padding_mode="border", align_corners=False)
+    out = F.grid_sample(mp[None, None], g.reshape(1, 1, n, 2), mode=mode,
+                        padding_mode=padding_mode, align_corners=align_corners)
     return out.view(coords_3d.shape[:-1])
 
 

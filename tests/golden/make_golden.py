@@ -257,6 +257,28 @@ def main():
                                                    src_ind=torch.tensor(ind), bins_dst=4, bins_src=4,
                                                    constant_marginal=False).numpy()
 
+    # MasterPattern.interpolate in every grid_sample mode (master_pattern.py:141-209;
+    # refining.py:203-209 and :338-344 call it with align_corners=True, bilinear and bicubic)
+    rng2 = np.random.default_rng(77)
+    mpm = band_master_pattern(R, rng2, 17)
+    mpm = (mpm + 0.2 * rng2.random(mpm.shape)).astype(np.float32)
+    out["interp_modes_mp"] = mpm
+    pts = unit_vecs(rng2, 1500)
+    ang = rng2.uniform(0, 2 * np.pi, 300)
+    rim = np.stack([np.cos(ang), np.sin(ang), np.zeros_like(ang)], 1)  # the square's edges
+    t = rng2.uniform(-1, 1, 200)
+    diag = np.stack([t, t * rng2.choice([-1, 1], 200), rng2.uniform(-1, 1, 200)], 1)
+    pts = np.concatenate([pts, rim, diag]).astype(np.float32)
+    pts *= rng2.uniform(0.5, 2.0, (len(pts), 1)).astype(np.float32)  # normalize_coords=True rescales
+    out["interp_modes_pts"] = pts
+    MPm = R["master_pattern"].MasterPattern(torch.tensor(mpm), laue_group=11)
+    for mode in ("bilinear", "nearest", "bicubic"):
+        for pad in ("zeros", "border", "reflection"):
+            for align in (False, True):
+                out[f"interp_{mode}_{pad}_{int(align)}"] = MPm.interpolate(
+                    torch.tensor(pts), mode=mode, padding_mode=pad, align_corners=align,
+                    normalize_coords=True).numpy()
+
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")
 

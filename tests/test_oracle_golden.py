@@ -204,6 +204,97 @@ def test_interpolate_matches_reference(golden):
     np.testing.assert_array_equal(got, golden["interp_out"])
 
 
+MODES = [(m, p, a) for m in ("bilinear", "nearest", "bicubic") for p in ("zeros", "border", "reflection")
+         for a in (False, True)]
+
+
+@pytest.mark.parametrize("mode,pad,align", MODES)
+def test_interpolate_modes_match_reference(golden, mode, pad, align):
+    got = O.interpolate(T(golden["interp_modes_mp"]), T(golden["interp_modes_pts"]), normalize_coords=True,
+                        mode=mode, padding_mode=pad, align_corners=align).numpy()
+    np.testing.assert_array_equal(got, golden[f"interp_{mode}_{pad}_{int(align)}"])
+
+
+def _source_index(g, size, align):
+    return (g + 1) / 2 * (size - 1) if align else ((g + 1) * size - 1) / 2
+
+
+def _pad_index(x, size, pad, align):
+    """Continuous source coordinate after padding (None: zeros padding)."""
+    if pad == "border":
+        return min(max(x, 0.0), size - 1)
+    if pad == "reflection":
+        lo, hi = (0.0, size - 1.0) if align else (-0.5, size - 0.5)
+        span = hi - lo
+        if span == 0:
+            return 0.0
+        d = abs(x - lo) % (2 * span)
+        x = lo + (d if d <= span else 2 * span - d)
+        return min(max(x, 0.0), size - 1)
+    return x
+
+
+def _tap(img, xi, yi):
+    H, W = img.shape
+    return img[yi, xi] if 0 <= xi < W and 0 <= yi < H else 0.0
+
+
+def _cubic(t, A=-0.75):
+    t = abs(t)
+    if t <= 1:
+        return ((A + 2) * t - (A + 3)) * t * t + 1
+    if t < 2:
+        return ((A * t - 5 * A) * t + 8 * A) * t - 4 * A
+    return 0.0
+
+
+def _brute_grid_sample(img, gx, gy, mode, pad, align):
+    """grid_sample from its definitions: source index, padding of the
+    coordinate (bilinear, nearest) or of each tap (bicubic), taps outside the
+    image read as 0, Keys' cubic convolution with A = -0.75."""
+    H, W = img.shape
+    out = np.empty(len(gx))
+    for n in range(len(gx)):
+        x = _source_index(gx[n], W, align)
+        y = _source_index(gy[n], H, align)
+        if mode == "bicubic":
+            x0, y0 = math.floor(x), math.floor(y)
+            acc = 0.0
+            for j in range(-1, 3):
+                for i in range(-1, 3):
+                    xi, yi = _pad_index(x0 + i, W, pad, align), _pad_index(y0 + j, H, pad, align)
+                    acc += _cubic(x - (x0 + i)) * _cubic(y - (y0 + j)) * _tap(img, int(xi), int(yi)) \
+                        if float(xi).is_integer() and float(yi).is_integer() else 0.0
+            out[n] = acc
+            continue
+        x, y = _pad_index(x, W, pad, align), _pad_index(y, H, pad, align)
+        if mode == "nearest":
+            out[n] = _tap(img, int(np.rint(x)), int(np.rint(y)))
+            continue
+        x0, y0 = math.floor(x), math.floor(y)
+        acc = 0.0
+        for yy, wy in ((y0, 1 - (y - y0)), (y0 + 1, y - y0)):
+            for xx, wx in ((x0, 1 - (x - x0)), (x0 + 1, x - x0)):
+                acc += wy * wx * _tap(img, xx, yy)
+        out[n] = acc
+    return out
+
+
+@pytest.mark.parametrize("mode,pad,align", MODES)
+def test_grid_sample_modes_brute_force(mode, pad, align):
+    """The library routine the oracle calls, in every mode, against the
+    definitions (float64; points up to 20% outside [-1, 1])."""
+    g = torch.Generator().manual_seed(11)
+    img = torch.rand(5, 10, generator=g, dtype=torch.float64)
+    pts = torch.rand(300, 2, generator=g, dtype=torch.float64) * 2.4 - 1.2
+    if mode == "nearest":  # keep clear of rounding ties
+        pts = pts + 1e-7
+    out = F.grid_sample(img[None, None], pts.view(1, 1, -1, 2), mode=mode, padding_mode=pad,
+                        align_corners=align).view(-1).numpy()
+    brute = _brute_grid_sample(img.numpy(), pts[:, 0].numpy(), pts[:, 1].numpy(), mode, pad, align)
+    np.testing.assert_allclose(out, brute, atol=1e-12)
+
+
 # ---------------------------------------------------------------- top-k
 
 
