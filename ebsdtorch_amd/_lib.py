@@ -81,6 +81,9 @@ _SIGNATURES = {
     "ebsd_sample_dirs_mode": (
         _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, ctypes.c_longlong, _c_void_p, _c_int, _c_int, _c_int,
                  _c_int, _c_void_p]),
+    "ebsd_sample_dirs_grad": (
+        _c_int, [_c_void_p, _c_int, _c_int, _c_void_p, ctypes.c_longlong, _c_void_p, _c_void_p, _c_int, _c_int,
+                 _c_int, _c_void_p]),
     "ebsd_prep_patterns": (
         _c_int,
         [_c_void_p, _c_int, _c_int, _c_void_p, _c_int, _c_void_p, _c_int, _c_int,

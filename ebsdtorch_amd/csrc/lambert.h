@@ -68,6 +68,44 @@ __device__ __forceinline__ void lambert_sbs(float x, float y, float z, float& gx
   gx = ((u * 0.5f) - 0.5f) + ((z < 0.0f) ? 1.0f : 0.0f);
 }
 
+// lambert_sbs with its Jacobian: j[0..2] = d gx / d(x, y, z), j[3..5] =
+// d gy / d(x, y, z), the derivatives autograd takes through the reference's
+// torch expressions (sphere.py:161-185): factor = sqrt(2 (1 - |z|)),
+// atan2(sml sgn, |big|) with d/d sml = big / (big^2 + sml^2) and
+// d/d big = -sml / (big^2 + sml^2). At the poles (factor 0, or big = 0) the
+// derivatives are taken as 0 where autograd would give inf or NaN.
+__device__ __forceinline__ void lambert_sbs_jac(float x, float y, float z, float& gx, float& gy, float (&j)[6]) {
+  lambert_sbs(x, y, z, gx, gy);
+  const float factor = __builtin_amdgcn_sqrtf(fmaxf(2.0f * (1.0f - fabsf(z)), 0.0f));
+  const bool cond = fabsf(y) <= fabsf(x);
+  const float big = cond ? x : y;
+  const float sml = cond ? y : x;
+  const float sgn = (big < 0.0f) ? -1.0f : 1.0f;
+  const float abig = fabsf(big);
+  const float ratio = abig > 0.0f ? (sml * sgn) / abig : 0.0f;
+  const float c1 = 0.70710678118654752440f, c2 = 0.90031631615710606956f;
+  const float zsign = z > 0.0f ? 1.0f : (z < 0.0f ? -1.0f : 0.0f);
+  const float dfdz = factor > 0.0f ? -zsign / factor : 0.0f;
+  const float r2 = big * big + sml * sml;
+  const float d_sml = r2 > 0.0f ? sgn * factor * c2 * big / r2 : 0.0f;
+  const float d_big = r2 > 0.0f ? -sgn * factor * c2 * sml / r2 : 0.0f;
+  const float simpler_z = sgn * c1 * dfdz;
+  const float arct_z = sgn * c2 * atan_unit(ratio) * dfdz;
+  float du[3], dv[3];
+  if (cond) {  // big = x, sml = y: u = simpler, v = arct
+    du[0] = 0.0f; du[1] = 0.0f; du[2] = simpler_z;
+    dv[0] = d_big; dv[1] = d_sml; dv[2] = arct_z;
+  } else {     // big = y, sml = x: u = arct, v = simpler
+    du[0] = d_sml; du[1] = d_big; du[2] = arct_z;
+    dv[0] = 0.0f; dv[1] = 0.0f; dv[2] = simpler_z;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    j[i] = 0.5f * du[i];
+    j[3 + i] = dv[i];
+  }
+}
+
 // Bilinear sample of an (H, W) image at normalised (gx, gy) with border
 // padding and align_corners=False: source index ((g + 1) * size - 1) / 2,
 // clipped to [0, size - 1]; the four taps are blended as

@@ -340,6 +340,33 @@ struct SampleAxis {
     else return v;
   }
   __device__ bool inside(int i) const { return i > -1 && i < size; }
+  // pad() with the derivative of its result by its argument, as torch's
+  // grid_sample backward takes it: clipping gives 0 on and beyond the
+  // borders, reflection -1 where the coordinate was folded an odd number of
+  // times
+  __device__ float pad_grad(float v, float& d) const {
+#pragma clang fp contract(off)
+    d = 1.0f;
+    if constexpr (PAD == EBSD_PAD_REFLECTION) {
+      if (twice_span == 0.0f) {
+        d = 0.0f;
+        return 0.0f;
+      }
+      const float rel = v - low;
+      const float a = fabsf(rel);
+      const float flips = truncf(a / twice_span);
+      const float extra = a - flips * twice_span;
+      const float back = twice_span - extra;
+      const bool more = extra > back;
+      d = (more != (rel < 0.0f)) ? -1.0f : 1.0f;
+      v = (more ? back : extra) + low;
+    }
+    if constexpr (PAD != EBSD_PAD_ZEROS) {
+      if (!(v > 0.0f && v < max_val)) d = 0.0f;
+      v = clip(v);
+    }
+    return v;
+  }
 };
 
 template <int PAD, bool ALIGN>
@@ -398,6 +425,82 @@ __device__ float sample_mode(const float* __restrict__ img, int H, int W, float 
              sample_tap(img, ax, ay, ix + 1, iy + 1) * (n * w);
     }
   }
+}
+
+// d/dt of the four cubic-convolution weights of cubic_coefficients(t)
+__device__ __forceinline__ void cubic_coefficients_grad(float t, float (&c)[4]) {
+  const float A = -0.75f;
+  auto d_near = [&](float x) { return (3.0f * (A + 2.0f) * x - 2.0f * (A + 3.0f)) * x; };  // |x| <= 1
+  auto d_far = [&](float x) { return (3.0f * A * x - 10.0f * A) * x + 8.0f * A; };         // 1 < |x| < 2
+  c[0] = d_far(t + 1.0f);
+  c[1] = d_near(t);
+  c[2] = -d_near(1.0f - t);
+  c[3] = -d_far(2.0f - t);
+}
+
+// Derivatives of sample_mode's value by gx and gy (0 for nearest).
+template <int MODE, int PAD, bool ALIGN>
+__device__ void sample_mode_grad(const float* __restrict__ img, int H, int W, float gx, float gy, float& dgx,
+                                 float& dgy) {
+  const SampleAxis<PAD, ALIGN> ax(W), ay(H);
+  dgx = 0.0f;
+  dgy = 0.0f;
+  if constexpr (MODE == EBSD_SAMPLE_BICUBIC) {
+    const float x = ax.unnormalize(gx), y = ay.unnormalize(gy);
+    const float fx = floorf(x), fy = floorf(y);
+    float cx[4], cy[4], dx[4], dy[4];
+    cubic_coefficients(x - fx, cx);
+    cubic_coefficients(y - fy, cy);
+    cubic_coefficients_grad(x - fx, dx);
+    cubic_coefficients_grad(y - fy, dy);
+    int xi[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) xi[i] = (int)ax.pad(fx + (float)(i - 1));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int yi = (int)ay.pad(fy + (float)(j - 1));
+      float row = 0.0f, drow = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float v = sample_tap(img, ax, ay, xi[i], yi);
+        row += cx[i] * v;
+        drow += dx[i] * v;
+      }
+      dgx += cy[j] * drow;
+      dgy += dy[j] * row;
+    }
+    dgx *= ax.scale;
+    dgy *= ay.scale;
+  } else if constexpr (MODE == EBSD_SAMPLE_BILINEAR) {
+    float mx, my;
+    const float x = ax.pad_grad(ax.unnormalize(gx), mx), y = ay.pad_grad(ay.unnormalize(gy), my);
+    const float x_w = floorf(x), y_n = floorf(y);
+    const float w = x - x_w, e = 1.0f - w, n = y - y_n, s = 1.0f - n;
+    const int ix = (int)x_w, iy = (int)y_n;
+    const float v_nw = sample_tap(img, ax, ay, ix, iy), v_ne = sample_tap(img, ax, ay, ix + 1, iy);
+    const float v_sw = sample_tap(img, ax, ay, ix, iy + 1), v_se = sample_tap(img, ax, ay, ix + 1, iy + 1);
+    dgx = (s * (v_ne - v_nw) + n * (v_se - v_sw)) * (mx * ax.scale);
+    dgy = (e * (v_sw - v_nw) + w * (v_se - v_ne)) * (my * ay.scale);
+  }
+}
+
+// Value and gradient by the direction (normalize_coords=False semantics: the
+// direction is used as given) of the master-pattern lookup.
+template <int MODE, int PAD, bool ALIGN>
+__global__ __launch_bounds__(256) void sample_dirs_grad_kernel(const float* __restrict__ mp, int H, int W,
+                                                               const float* __restrict__ dirs, int64_t n,
+                                                               float* __restrict__ out,
+                                                               float* __restrict__ grad) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+  float gx, gy, jac[6];
+  lambert_sbs_jac(x, y, z, gx, gy, jac);
+  out[i] = sample_mode<MODE, PAD, ALIGN>(mp, H, W, gx, gy);
+  float dgx, dgy;
+  sample_mode_grad<MODE, PAD, ALIGN>(mp, H, W, gx, gy, dgx, dgy);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) grad[3 * i + c] = dgx * jac[c] + dgy * jac[3 + c];
 }
 
 template <int MODE, int PAD, bool ALIGN>
@@ -640,30 +743,46 @@ extern "C" int ebsd_sample_dirs(const float* mp, int mp_rows, int mp_cols, const
   return EBSD_OK;
 }
 
-template <int MODE, int PAD>
-static void launch_sample_mode(bool align, dim3 grid, hipStream_t st, const float* mp, int H, int W,
-                               const float* dirs, int64_t n, float* out, int renorm) {
-  if (align)
-    hipLaunchKernelGGL((sample_dirs_mode_kernel<MODE, PAD, true>), grid, dim3(256), 0, st, mp, H, W, dirs, n, out,
-                       renorm);
+// Launch of the value kernel (grad == nullptr: renorm applies) or the
+// value + gradient kernel for one (mode, padding, alignment).
+template <int MODE, int PAD, bool ALIGN>
+static void launch_sample_one(dim3 grid, hipStream_t st, const float* mp, int H, int W, const float* dirs,
+                              int64_t n, float* out, int renorm, float* grad) {
+  if (grad)
+    hipLaunchKernelGGL((sample_dirs_grad_kernel<MODE, PAD, ALIGN>), grid, dim3(256), 0, st, mp, H, W, dirs, n, out,
+                       grad);
   else
-    hipLaunchKernelGGL((sample_dirs_mode_kernel<MODE, PAD, false>), grid, dim3(256), 0, st, mp, H, W, dirs, n, out,
+    hipLaunchKernelGGL((sample_dirs_mode_kernel<MODE, PAD, ALIGN>), grid, dim3(256), 0, st, mp, H, W, dirs, n, out,
                        renorm);
+}
+
+template <int MODE, int PAD>
+static void launch_sample_align(bool align, dim3 grid, hipStream_t st, const float* mp, int H, int W,
+                                const float* dirs, int64_t n, float* out, int renorm, float* grad) {
+  if (align)
+    launch_sample_one<MODE, PAD, true>(grid, st, mp, H, W, dirs, n, out, renorm, grad);
+  else
+    launch_sample_one<MODE, PAD, false>(grid, st, mp, H, W, dirs, n, out, renorm, grad);
 }
 
 template <int MODE>
 static void launch_sample_pad(int padding, bool align, dim3 grid, hipStream_t st, const float* mp, int H, int W,
-                              const float* dirs, int64_t n, float* out, int renorm) {
+                              const float* dirs, int64_t n, float* out, int renorm, float* grad) {
   switch (padding) {
-    case EBSD_PAD_ZEROS: launch_sample_mode<MODE, EBSD_PAD_ZEROS>(align, grid, st, mp, H, W, dirs, n, out, renorm); break;
-    case EBSD_PAD_BORDER: launch_sample_mode<MODE, EBSD_PAD_BORDER>(align, grid, st, mp, H, W, dirs, n, out, renorm); break;
-    default: launch_sample_mode<MODE, EBSD_PAD_REFLECTION>(align, grid, st, mp, H, W, dirs, n, out, renorm); break;
+    case EBSD_PAD_ZEROS:
+      launch_sample_align<MODE, EBSD_PAD_ZEROS>(align, grid, st, mp, H, W, dirs, n, out, renorm, grad);
+      break;
+    case EBSD_PAD_BORDER:
+      launch_sample_align<MODE, EBSD_PAD_BORDER>(align, grid, st, mp, H, W, dirs, n, out, renorm, grad);
+      break;
+    default:
+      launch_sample_align<MODE, EBSD_PAD_REFLECTION>(align, grid, st, mp, H, W, dirs, n, out, renorm, grad);
+      break;
   }
 }
 
-extern "C" int ebsd_sample_dirs_mode(const float* mp, int mp_rows, int mp_cols, const float* dirs, long long n,
-                                     float* out, int renorm, int mode, int padding, int align_corners,
-                                     void* stream) {
+static int sample_dirs_any(const float* mp, int mp_rows, int mp_cols, const float* dirs, long long n, float* out,
+                           int renorm, int mode, int padding, int align_corners, float* grad, void* stream) {
   if (!mp || !dirs || !out) return EBSD_ERR_NULL;
   if (mp_rows < 1 || mp_cols < 1 || n < 0) return EBSD_ERR_SHAPE;
   if ((int64_t)mp_rows * mp_cols > (int64_t)INT_MAX) return EBSD_ERR_SHAPE;
@@ -675,19 +794,33 @@ extern "C" int ebsd_sample_dirs_mode(const float* mp, int mp_rows, int mp_cols, 
   const dim3 grid((unsigned)blocks);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool align = align_corners != 0;
+  const int H = mp_rows, W = mp_cols;
   switch (mode) {
     case EBSD_SAMPLE_BILINEAR:
-      launch_sample_pad<EBSD_SAMPLE_BILINEAR>(padding, align, grid, st, mp, mp_rows, mp_cols, dirs, n, out, renorm);
+      launch_sample_pad<EBSD_SAMPLE_BILINEAR>(padding, align, grid, st, mp, H, W, dirs, n, out, renorm, grad);
       break;
     case EBSD_SAMPLE_NEAREST:
-      launch_sample_pad<EBSD_SAMPLE_NEAREST>(padding, align, grid, st, mp, mp_rows, mp_cols, dirs, n, out, renorm);
+      launch_sample_pad<EBSD_SAMPLE_NEAREST>(padding, align, grid, st, mp, H, W, dirs, n, out, renorm, grad);
       break;
     default:
-      launch_sample_pad<EBSD_SAMPLE_BICUBIC>(padding, align, grid, st, mp, mp_rows, mp_cols, dirs, n, out, renorm);
+      launch_sample_pad<EBSD_SAMPLE_BICUBIC>(padding, align, grid, st, mp, H, W, dirs, n, out, renorm, grad);
       break;
   }
   EBSD_LAUNCH_CHECK();
   return EBSD_OK;
+}
+
+extern "C" int ebsd_sample_dirs_mode(const float* mp, int mp_rows, int mp_cols, const float* dirs, long long n,
+                                     float* out, int renorm, int mode, int padding, int align_corners,
+                                     void* stream) {
+  return sample_dirs_any(mp, mp_rows, mp_cols, dirs, n, out, renorm, mode, padding, align_corners, nullptr, stream);
+}
+
+extern "C" int ebsd_sample_dirs_grad(const float* mp, int mp_rows, int mp_cols, const float* dirs, long long n,
+                                     float* out, float* grad, int mode, int padding, int align_corners,
+                                     void* stream) {
+  if (!grad) return EBSD_ERR_NULL;
+  return sample_dirs_any(mp, mp_rows, mp_cols, dirs, n, out, 0, mode, padding, align_corners, grad, stream);
 }
 
 extern "C" int ebsd_prep_patterns(const float* pats, int n, int ld_in, const int* pix_idx,

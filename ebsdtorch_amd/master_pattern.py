@@ -117,6 +117,12 @@ class MasterPattern(torch.nn.Module):
                              f"'reflection', but got: '{padding_mode}'")
         require_device(coords_3D, "coords_3D")
         mp = self.pattern_for_binning(virtual_binning)
+        if coords_3D.requires_grad and torch.is_grad_enabled():
+            # differentiable in the directions, as grid_sample is in its grid
+            c = coords_3D / torch.norm(coords_3D, dim=-1, keepdim=True) if normalize_coords else coords_3D
+            out = _SampleDirs.apply(c.reshape(-1, 3).float().contiguous(), mp, _lib.SAMPLE_MODES[mode],
+                                    _lib.PADDING_MODES[padding_mode], bool(align_corners))
+            return out.view(coords_3D.shape[:-1])
         dirs = coords_3D.reshape(-1, 3).float().contiguous()
         out = torch.empty(dirs.shape[0], dtype=torch.float32, device=dirs.device)
         lib = _lib.load()
@@ -132,3 +138,26 @@ class MasterPattern(torch.nn.Module):
                                            _lib.PADDING_MODES[padding_mode], 1 if align_corners else 0, stream)
             check(st, "ebsd_sample_dirs_mode")
         return out.view(coords_3D.shape[:-1])
+
+
+class _SampleDirs(torch.autograd.Function):
+    """Master-pattern lookup (n, 3) -> (n,) with its gradient by the
+    directions from ebsd_sample_dirs_grad (the derivative autograd takes
+    through the Lambert map and grid_sample; none for the master pattern,
+    which the reference keeps as a buffer)."""
+
+    @staticmethod
+    def forward(ctx, dirs, mp, mode, padding, align):
+        out = torch.empty(dirs.shape[0], dtype=torch.float32, device=dirs.device)
+        grad = torch.empty_like(dirs)
+        st = _lib.load().ebsd_sample_dirs_grad(mp.data_ptr(), mp.shape[0], mp.shape[1], dirs.data_ptr(),
+                                               dirs.shape[0], out.data_ptr(), grad.data_ptr(), mode, padding,
+                                               1 if align else 0, _lib.stream_ptr(dirs.device))
+        check(st, "ebsd_sample_dirs_grad")
+        ctx.save_for_backward(grad)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (grad,) = ctx.saved_tensors
+        return g[:, None] * grad, None, None, None, None

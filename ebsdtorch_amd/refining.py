@@ -22,7 +22,7 @@ from ._lib import DTYPE_F32
 from .experiment_pats import ExperimentPatterns
 from .geometry import EBSDGeometry
 from .master_pattern import MasterPattern
-from .orientations import cu2qu, get_local_orientation_grid, laue_elements
+from .orientations import cu2qu, get_local_orientation_grid, laue_elements, ori_to_fz_laue, qu_apply
 
 
 def _grid_levels(semi_edge_in_degrees: float, kernel_radius_in_steps: int, axial_grid_dimension: int,
@@ -101,3 +101,70 @@ def orientation_grid_refinement(
             experiment_patterns.set_orientations(q, batch)
         all_dots.append(dots)
     return all_dots
+
+
+def orientation_grad_refinement(
+    master_patterns: Union[MasterPattern, List[MasterPattern]],
+    geometry: EBSDGeometry,
+    experiment_patterns: ExperimentPatterns,
+    batch_size: int,
+    virtual_binning: int = 1,
+    n_iter: int = 50,
+    learning_rate: float = 0.001,
+    average_pattern_center: bool = True,
+    match_dtype: torch.dtype = torch.float16,
+) -> None:
+    """Refine orientations by gradient ascent on the cosine (refining.py:247-388),
+    in place: per batch, Adam on the quaternions for n_iter steps; the loss
+    is minus the mean cosine between each zero-mean unit pattern and its
+    bicubic (align_corners=True) simulation, whose derivative by the
+    directions comes from ebsd_sample_dirs_grad; the rest (rotation, means,
+    norms, the optimizer) is torch on the patterns' device, as in the
+    reference. Afterwards the quaternions go to the positive hemisphere and
+    the fundamental zone and are written back only where the final dot is
+    below minus the first one (refining.py:358-359, 381-388), which a
+    matched pattern (positive dots) never satisfies: the reference's rule,
+    kept as it is. The dots are taken from `match_dtype` operands with FP32
+    accumulation, rounded to `match_dtype`."""
+    if not average_pattern_center:
+        raise NotImplementedError("per-pattern pattern centres are not supported; use average_pattern_center=True")
+    if not hasattr(experiment_patterns, "orientations") or experiment_patterns.orientations is None:
+        raise ValueError("The experiment patterns must be indexed before refinement.")
+    if isinstance(master_patterns, MasterPattern):
+        master_patterns = [master_patterns]
+    dev = experiment_patterns.patterns.device
+    if dev.type != "cuda":
+        raise ValueError("gradient refinement runs on the GPU: the experiment patterns must be on a CUDA device")
+    rays = geometry.get_coords_sample_frame(binning=(virtual_binning, virtual_binning)).view(-1, 3)
+    rays = (rays / rays.norm(dim=-1, keepdim=True)).to(dev, torch.float32)
+    for phase, indices in enumerate(experiment_patterns.get_indices_per_phase()):
+        mp = master_patterns[phase]
+        for batch in torch.split(indices.reshape(-1).to(dev), batch_size):
+            pats = experiment_patterns.get_patterns(batch, binning=virtual_binning)
+            pats = pats.reshape(pats.shape[0], -1).to(torch.float32)
+            pats = pats - torch.mean(pats, dim=-1, keepdim=True)
+            pats = pats / pats.norm(dim=-1, keepdim=True)
+            q = experiment_patterns.get_orientations(batch).to(dev, torch.float32).clone().detach()
+            q.requires_grad_(True)
+            opt = torch.optim.Adam([q], lr=learning_rate)
+            initial = None
+            for step in range(n_iter):
+                rot = qu_apply(q[:, None, :], rays[None, :, :])
+                sim = mp.interpolate(rot, mode="bicubic", align_corners=True, normalize_coords=False,
+                                     virtual_binning=virtual_binning).reshape(q.shape[0], -1)
+                sim = sim - torch.mean(sim, dim=-1, keepdim=True)
+                sim = sim / sim.norm(dim=-1, keepdim=True)
+                dots = (pats.to(match_dtype).float() * sim.to(match_dtype).float()).sum(-1).to(match_dtype)
+                if step == 0:
+                    initial = -dots
+                loss = -torch.mean(dots)
+                opt.zero_grad()
+                loss.backward()
+                opt.step()
+                with torch.no_grad():
+                    q /= q.norm(dim=-1, keepdim=True)
+            q = q.detach()
+            q = q * ((q[..., :1] >= 0).float() * 2 - 1)
+            q = ori_to_fz_laue(q, mp.laue_group)
+            upd = dots < initial
+            experiment_patterns.set_orientations(q[upd], batch[upd])

@@ -162,6 +162,20 @@ int ebsd_sample_dirs_mode(const float* mp, int mp_rows, int mp_cols, const float
                           float* out, int renorm, int mode, int padding, int align_corners, void* stream);
 
 /*
+ * ebsd_sample_dirs_mode without re-normalisation, with the gradient of each
+ * value by its direction: grad[i] = d out[i] / d dirs[i] (3 floats), through
+ * the Lambert map and the sampling as torch's autograd differentiates
+ * grid_sample (nearest: 0; border and reflection: 0 on and beyond the
+ * borders).
+ *   dirs (n, 3) f32; out (n,) f32; grad (n, 3) f32
+ * Replaces: the backward of ebsd/master_pattern.py:141-209 that
+ * refining.py:318-365 (orientation_grad_refinement) takes through
+ * MasterPattern.interpolate.
+ */
+int ebsd_sample_dirs_grad(const float* mp, int mp_rows, int mp_cols, const float* dirs, long long n,
+                          float* out, float* grad, int mode, int padding, int align_corners, void* stream);
+
+/*
  * Experimental-pattern preparation: gather pixels (pix_idx may be NULL for
  * the identity), optionally zero-mean / unit-norm each row, convert to
  * out_dtype and zero-pad to ld_out.

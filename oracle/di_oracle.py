@@ -775,3 +775,53 @@ def orientation_grid_refinement(mp: torch.Tensor, laue_id: int, rays: torch.Tens
             cu_cur[best_i == ((cu_cur.shape[1] - 1) / 2)] *= shrink_factor
         out[idx] = q_cur
     return out, dots
+
+
+def orientation_grad_refinement(mp: torch.Tensor, laue_id: int, rays: torch.Tensor, exp_pats: torch.Tensor,
+                                orientations: torch.Tensor, batch_size: int, virtual_binning: int = 1,
+                                n_iter: int = 50, learning_rate: float = 0.001,
+                                match_dtype=torch.float16) -> torch.Tensor:
+    """orientation_grad_refinement (refining.py:247-388) for one phase with an
+    average pattern centre: Adam on the quaternions, the loss minus the mean
+    cosine between each zero-mean unit pattern and its bicubic
+    (align_corners=True) simulation, back-propagated by autograd through
+    grid_sample; after n_iter steps the quaternions go to the positive
+    hemisphere and the fundamental zone, and are kept only where the final
+    dot is below minus the first one (refining.py:381-388). mp (S, 2S)
+    unbinned; rays (D_b, 3) unit sample-frame directions of the binned
+    detector; exp_pats (N, H, W); orientations (N, 4). Returns (N, 4)."""
+    mp_b = mp
+    if virtual_binning > 1:
+        mp_b = F.avg_pool2d(mp[None, None], virtual_binning, stride=virtual_binning)[0, 0]
+    n = exp_pats.shape[0]
+    out = orientations.clone()
+    for idx in torch.split(torch.arange(n), batch_size):
+        pats = exp_pats[idx]
+        if virtual_binning > 1:
+            pats = F.avg_pool2d(pats[:, None], virtual_binning)[:, 0]
+        pats = pats.reshape(len(idx), -1)
+        pats = pats - torch.mean(pats, dim=-1, keepdim=True)
+        pats = pats / pats.norm(dim=-1, keepdim=True)
+        q = out[idx].clone().detach().requires_grad_(True)
+        opt = torch.optim.Adam([q], lr=learning_rate)
+        initial = None
+        for step in range(n_iter):
+            rot = qu_apply(q[:, None, :], rays[None, :, :])
+            sim = interpolate(mp_b, rot, normalize_coords=False, mode="bicubic", align_corners=True).squeeze()
+            sim = sim - torch.mean(sim, dim=-1, keepdim=True)
+            sim = sim / sim.norm(dim=-1, keepdim=True)
+            d = torch.einsum("ij,ij->i", pats.to(match_dtype), sim.to(match_dtype))
+            if step == 0:
+                initial = -d
+            loss = -torch.mean(d)
+            opt.zero_grad()
+            loss.backward()
+            opt.step()
+            with torch.no_grad():
+                q /= q.norm(dim=-1, keepdim=True)
+        q = q.detach()
+        q *= (q[..., :1] >= 0).float() * 2 - 1
+        q = ori_to_fz_laue(q, laue_id)
+        upd = d < initial
+        out[idx[upd]] = q[upd]
+    return out

@@ -279,6 +279,22 @@ def main():
                     torch.tensor(pts), mode=mode, padding_mode=pad, align_corners=align,
                     normalize_coords=True).numpy()
 
+    # Gradient refinement (ebsd/refining.py:247-388): the grid-refinement
+    # problem above (33x66 MP, 8x8 detector), 16 patterns from 1.5 degrees
+    # off; the last four with inverted contrast (anti-matches: the only
+    # patterns whose orientations the function writes back); 10 Adam steps
+    geom8 = R["geometry"].EBSDGeometry((8, 8), proj_center=(0.5, 0.5, 0.6))
+    gpats = out["di_exp"].copy()
+    gpats[12:] = gpats[12:].max() - gpats[12:]
+    out["grad_pats"] = gpats
+    for tag, md in (("f32", torch.float32), ("f16", torch.float16)):
+        ep = R["experiment_pats"].ExperimentPatterns(torch.tensor(gpats))
+        ep.set_orientations(torch.tensor(out["refine_start"]).clone())
+        ep.phases = torch.zeros(16, dtype=torch.uint8)
+        MPg = R["master_pattern"].MasterPattern(torch.tensor(out["mp33"]), laue_group=11)
+        R["refining"].orientation_grad_refinement(MPg, geom8, ep, batch_size=8, n_iter=10, match_dtype=md)
+        out[f"grad_{tag}_ori"] = ep.get_orientations().numpy()
+
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")
 
