@@ -71,7 +71,7 @@ def parse():
     p.add_argument("--learned-train", type=int, default=2_000_000,
                    help="training patterns of the learned-proposal form (0 = skip that form)")
     p.add_argument("--learned-batch", type=int, default=65536, help="patterns indexed by the learned form")
-    p.add_argument("--workload", default="metric", choices=["metric", "c3", "c5", "projection"],
+    p.add_argument("--workload", default="metric", choices=["metric", "c3", "c5", "projection", "library-gemm"],
                    help="metric: the headline configuration; c3: the precision forms and PCA DI at 80x80; "
                         "c5: 6/mmm, 120x120, 1M dictionary, streamed scan")
     p.add_argument("--proj-config", default="metric", choices=["metric", "c3", "c5"],
@@ -113,6 +113,9 @@ def main():
         return
     if a.workload == "projection":
         print(json.dumps(projection_workload(a, dev)), flush=True)
+        return
+    if a.workload == "library-gemm":
+        print(json.dumps(library_gemm_workload(a, dev)), flush=True)
         return
     if a.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
@@ -822,6 +825,56 @@ def projection_workload(a, dev):
                         f"({'m-3m' if laue == 11 else '6/mmm'} {a.mp_size}x{a.mp_size} master pattern)",
             "steps": a.steps, **out}
 
+
+
+def library_gemm_workload(a, dev):
+    """The similarity contraction as a plain library GEMM (hipBLASLt through
+    torch): 65,536 query rows x 16,384 dictionary rows per call at the row
+    widths the search kernels use (FP8 3,712 and 256 bytes, BF16 3,648
+    elements), BF16 output (2.1 GB per call, written; no top-k). A
+    reference point for the fused kernels, which compute the same products
+    and keep only the top 16 per query. HIP-event time per call over
+    `steps` calls after one warm-up."""
+    import torch
+
+    m, n = a.batch, 16384
+    gen = torch.Generator(device=dev).manual_seed(0)
+    out = {}
+    cases = [("fp8", 3712), ("fp8", 256), ("bf16", 3648)]
+    for name, k in cases:
+        x = torch.randn(m, k, device=dev, generator=gen)
+        y = torch.randn(n, k, device=dev, generator=gen)
+        try:
+            if name == "fp8":
+                x8, y8 = x.to(torch.float8_e4m3fn), y.to(torch.float8_e4m3fn)
+                one = torch.ones((), device=dev)
+
+                def run():
+                    return torch._scaled_mm(x8, y8.t(), scale_a=one, scale_b=one, out_dtype=torch.bfloat16)
+            else:
+                xb, yb = x.to(torch.bfloat16), y.to(torch.bfloat16)
+
+                def run():
+                    return torch.matmul(xb, yb.t())
+            run()
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(a.steps):
+                run()
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1]) / a.steps
+            tflops = 2.0 * m * n * k / ms / 1e9
+            out[f"{name}_k{k}"] = {"ms": round(ms, 3), "TFLOPs": round(tflops, 1),
+                                   "frac": round(tflops / PEAKS[name], 4)}
+            log(f"library GEMM {name} {m}x{n}x{k}: {ms:.3f} ms, {tflops:.1f} TFLOP/s")
+        except Exception as e:  # report an unsupported case instead of failing the run
+            out[f"{name}_k{k}"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"library GEMM {name} k={k}: {type(e).__name__}: {e}")
+        torch.cuda.empty_cache()
+    return {"workload": f"library GEMM (hipBLASLt via torch) {m} x {n} per call, BF16 output", "steps": a.steps,
+            **out}
 
 
 def c3_workload(a, dev):
