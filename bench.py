@@ -241,9 +241,10 @@ def main():
     # profiles/ (null when the workload differs from the measured one)
     traffic = lookup_traffic(a.precision, n_dict, a.batch, [det, det], a.topk, search_dim, D)
 
-    cpu = None
+    cpu = c1 = None
     if a.cpu_seconds > 0:
         cpu = cpu_baseline(mp.cpu(), rays.cpu(), exp[:1024].cpu(), dico.cpu(), n_dict, a.cpu_seconds)
+        c1 = c1_form(a, rays, dev)
 
     build_s = build_ms[-1] / 1000
     res = {
@@ -310,6 +311,7 @@ def main():
         "nlpar": nlpar_res,
         "pc_fit": pcfit,
         "learned_proposal": learned,
+        "c1": c1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
@@ -1143,6 +1145,54 @@ def c5_workload(a, dev):
                              "train": {"patterns": tlog.patterns, "seconds": round(tlog.seconds, 1),
                                        "loss_last": round(tlog.loss_curve[-1][1], 4)}},
     }
+
+
+def c1_form(a, rays, dev):
+    """The small configuration in full on both sides: 201x201 m-3m band
+    master pattern, the ~10k-entry dictionary (sample_ori_fz_laue(11,
+    10,000)), 1,000 noisy 60x60 patterns, top-1. GPU: plain FP32 DI
+    (DictionaryIndexer), with and without its dictionary build. CPU: the
+    oracle's dictionary_index (projection + FP32 matching at the
+    reference's 256 x 256 chunking) on all 1,000 patterns, and the int8
+    variant. Top-1 agreement of the two, with the disagreements' gaps."""
+    import torch
+
+    import ebsdtorch_amd as E
+    from ebsdtorch_amd import synthetic
+    from oracle import di_oracle as O
+
+    mp = synthetic.band_master_pattern(201, 11, 40, 2.0, seed=0, device=dev)
+    dico = E.sample_ori_fz_laue(11, 10_000, dev, permute=True, generator=torch.Generator().manual_seed(1))
+    ori = synthetic.random_orientations(1000, seed=2, device=dev)
+    exp = synthetic.simulate_patterns(mp, rays, ori, mean_counts=200.0, gaussian_sigma=0.05, seed=3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng = E.DictionaryIndexer(mp, rays, dico, precision="fp32")
+    gidx = form_top1(eng, exp)
+    torch.cuda.synchronize()
+    with_build = time.perf_counter() - t0
+    run = time_form(eng, exp, 1, 5, 1)
+    mp_c, rays_c, exp_c, dico_c = mp.cpu(), rays.cpu(), exp.cpu(), dico.cpu()
+    t0 = time.perf_counter()
+    cidx, _ = O.dictionary_index(mp_c, rays_c, exp_c, dico_c, top_k=1)
+    t_cpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.dictionary_index(mp_c, rays_c, exp_c, dico_c, top_k=1, quantized=True)
+    t_int8 = time.perf_counter() - t0
+    q32 = eng.prepare(exp)
+    rep = mismatch_report(q32, eng.dictionary, gidx, cidx[:, 0].to(dev).int())
+    agree = float((gidx.cpu().long() == cidx[:, 0]).float().mean())
+    log(f"C1: GPU FP32 DI {run['rate']:.0f} patterns/s ({1000 / with_build:.0f} with the build), CPU oracle "
+        f"{1000 / t_cpu:.1f} patterns/s (int8 {1000 / t_int8:.1f}), top-1 agreement {agree:.4f}")
+    return {"workload": f"C1: {dico.shape[0]} m-3m dictionary x 1000 patterns, 60x60, 201x201 master pattern, top-1",
+            "gpu_fp32_patterns_per_s": round(run["rate"], 1),
+            "gpu_fp32_patterns_per_s_with_build": round(1000 / with_build, 1),
+            "cpu_oracle_patterns_per_s": round(1000 / t_cpu, 2),
+            "cpu_oracle_int8_patterns_per_s": round(1000 / t_int8, 2),
+            "cpu_cores": torch.get_num_threads(),
+            "top1_agreement_gpu_vs_cpu_oracle": agree, "disagreements": rep,
+            "what": "the CPU figures include the oracle's own dictionary projection (as the reference's call "
+                    "does); the first GPU figure excludes the one-off dictionary build, the second includes it"}
 
 
 def cpu_baseline(mp, rays, exp, dico, n_dict, seconds):
