@@ -88,7 +88,10 @@ def _mi_maps(sim: torch.Tensor, src_fft: torch.Tensor, pshape, chunk: int = 128)
     """cmif(images_dst=sim, images_src=patterns, src_ind=detector area, 4 + 4
     bins) for dictionary patterns sim (D, 1, Hp, Wp): (D, B, Hp, Wp). The
     25 cross-correlations of the indicator and level channels are FFT
-    products; ebsd_cmif_maps turns them into entropies per shift."""
+    products; ebsd_cmif_maps turns them into entropies per shift. The
+    inverse transform is taken at the padded shape (the reference's cmif
+    leaves irfft2 its default size, one column short for odd widths, which
+    its fitter cannot then combine; equal for even widths)."""
     d = sim.shape[0]
     out = []
     for s0 in range(0, d, chunk):
@@ -96,7 +99,7 @@ def _mi_maps(sim: torch.Tensor, src_fft: torch.Tensor, pshape, chunk: int = 128)
         dst = torch.cat((torch.ones_like(x), _levels(x)), 1)
         dst_fft = torch.fft.rfft2(dst)  # (d, 5, Hp, Wf)
         prod = dst_fft[:, None, :, None] * src_fft[None, :, None, :]  # (d, B, 5, 5, Hp, Wf)
-        corr = torch.fft.irfft2(prod)
+        corr = torch.fft.irfft2(prod, s=tuple(pshape))
         del prod
         out.append(torch.fft.fftshift(kernels.cmif_maps(corr), dim=(-2, -1)))
         del corr

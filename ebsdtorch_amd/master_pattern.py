@@ -116,7 +116,7 @@ class MasterPattern(torch.nn.Module):
             raise ValueError(f"nn.functional.grid_sample(): expected padding_mode to be 'zeros', 'border', or "
                              f"'reflection', but got: '{padding_mode}'")
         require_device(coords_3D, "coords_3D")
-        mp = self.pattern_for_binning(virtual_binning)
+        mp = self.pattern_for_binning(virtual_binning).to(coords_3D.device)
         if coords_3D.requires_grad and torch.is_grad_enabled():
             # differentiable in the directions, as grid_sample is in its grid
             c = coords_3D / torch.norm(coords_3D, dim=-1, keepdim=True) if normalize_coords else coords_3D

@@ -51,6 +51,23 @@ def test_mi_maps_match_reference_cmif(golden):
     torch.testing.assert_close(got[keep], ref[keep], rtol=1e-4, atol=1e-5)
 
 
+def test_mi_maps_odd_width_against_brute_force(golden):
+    """An odd image width (12 x 9): the maps equal direct counting at every
+    shift (shifts where a histogram bin holds exactly one pair excluded, as
+    above)."""
+    from ebsdtorch_amd.fitting import _levels, _mi_maps
+
+    dst, src, ind = (torch.tensor(golden[k])[..., :9].contiguous() for k in ("cmif_dst", "cmif_src", "cmif_ind"))
+    h, w = src.shape[-2:]
+    src_fft = torch.fft.rfft2(torch.cat((ind, _levels(src)), 1).to(DEV)).conj()
+    got = _mi_maps(dst.to(DEV), src_fft, (h, w)).cpu()
+    ref, ones = FO.cmif_brute(dst, src, ind)
+    assert got.shape == ref.shape == (3, 2, h, w)
+    keep = ~ones
+    assert int(keep.sum()) > 100
+    torch.testing.assert_close(got[keep], ref[keep].float(), rtol=1e-4, atol=1e-5)
+
+
 def test_spatial_fft_pc_fitter_mi_matches_oracle():
     """The fitter with metric "mi" (fitting.py:486-496): the same centre and
     orientations as the oracle's mutual-information fit."""

@@ -87,3 +87,16 @@ def test_interpolate_rejects_unknown_modes():
         m.interpolate(pts, mode="area")
     with pytest.raises(ValueError):
         m.interpolate(pts, padding_mode="wrap")
+
+
+def test_interpolate_with_a_host_master_pattern():
+    """A MasterPattern kept on the host (as the reference builds it) samples
+    GPU directions: the buffer goes to the directions' device."""
+    from ebsdtorch_amd.master_pattern import MasterPattern
+
+    mp = torch.rand(21, 42, generator=torch.Generator().manual_seed(1))
+    pts = torch.randn(500, 3, generator=torch.Generator().manual_seed(2))
+    host = MasterPattern(mp, laue_group=11)
+    dev = MasterPattern(mp.to(DEV), laue_group=11)
+    for kw in ({}, {"mode": "bicubic", "align_corners": True}):
+        assert torch.equal(host.interpolate(pts.to(DEV), **kw), dev.interpolate(pts.to(DEV), **kw))
