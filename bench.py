@@ -278,7 +278,7 @@ def main():
             "build_s": round(build_s, 3),
             "patterns_per_s_one_batch": round(a.batch / (build_s + head["ms_per_step"] / 1000), 1),
             "what": "one batch indexed right after building the dictionary engine (projection, second moment, "
-                    "FP64 eigensolve, coordinates); the build is once per dictionary",
+                    "FP64 leading eigenvectors, coordinates); the build is once per dictionary",
         },
         "cosine_error_vs_fp64": None if head_cos is None else {
             a.precision + ("_pca" if pca_search else "") + "+rescore": head_cos,

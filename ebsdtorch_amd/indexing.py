@@ -86,7 +86,7 @@ class DictionaryIndexer(_ResidentSearch):
                  precision: str = "fp32", rescore_k: int = 0, dict_unit_norm: bool = False,
                  renorm_dirs: bool = False, search_components: Union[int, str] = 0,
                  explained_variance: float = 0.995, basis_sample: int = 1 << 18, rotate_basis: bool = True,
-                 exact_fp32: bool = True):
+                 exact_fp32: bool = True, basis_solver: str = "subspace"):
         if precision not in kernels.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(kernels.PRECISIONS)}")
         if search_components and rescore_k <= 0:
@@ -107,6 +107,9 @@ class DictionaryIndexer(_ResidentSearch):
         self.basis_t = None
         self.basis_t_bf16 = None
         self.rotate_basis = bool(rotate_basis)
+        if basis_solver not in ("subspace", "full"):
+            raise ValueError("basis_solver must be 'subspace' or 'full'")
+        self.basis_solver = basis_solver
         self.search_components = 0
         self.explained = 1.0
         if search_components:
@@ -130,11 +133,15 @@ class DictionaryIndexer(_ResidentSearch):
         approximated), in the search dtype; rescore_k > 0 re-scores them
         exactly against the full FP32 dictionary. The second moment is
         accumulated on the matrix cores (ebsd_cov_update: centred aggregate,
-        plus n mean mean^T) over up to `sample` dictionary rows and solved in
-        FP64 on the host (fp64_eigh). components "auto": the smallest multiple of 64 (at least the
-        search kernel's 4 K stages) whose leading eigenvalues hold
+        plus n mean mean^T) over up to `sample` dictionary rows. Its leading
+        eigenvectors come from FP64 subspace iteration (leading_eigh:
+        products on the device, the small Rayleigh-Ritz solve on the host;
+        basis_solver "subspace") or from the full FP64 solve on the host
+        (fp64_eigh; "full", and the fallback when the leading 512 do not hold
+        `explained`). components "auto": the smallest multiple of 64 (at
+        least the search kernel's 4 K stages) whose leading eigenvalues hold
         `explained` of the trace; an int: that many."""
-        from .pca_covmat import OnlineCovMatrix, fp64_eigh
+        from .pca_covmat import OnlineCovMatrix, fp64_eigh, leading_eigh
 
         full = self.dictionary_f32
         d = full.n_pix
@@ -145,19 +152,32 @@ class DictionaryIndexer(_ResidentSearch):
         n_obs = float(int(cov.obs.item()))
         mean = cov.mean.to(torch.float64)
         m2 = cov._full_aggregate().to(torch.float64) + n_obs * torch.outer(mean, mean)
-        # one FP64 solve per dictionary, on the host (fp64_eigh)
-        evals, evecs = fp64_eigh(m2)
-        evals = evals.clamp_min(0.0)
+        trace = float(torch.trace(m2))
         elem = kernels.TORCH_DTYPE[self.dtype].itemsize
         k_min = max(64, -(-256 // elem))  # >= 4 K stages of 64 bytes
+
+        def rounded(k):
+            return min(max(-(-k // 64) * 64, k_min), d)
+
+        # one FP64 solve per dictionary
         if components == "auto":
-            frac = torch.cumsum(evals.flip(0), 0) / evals.sum()
-            k = int(torch.searchsorted(frac, torch.tensor([explained], dtype=frac.dtype, device=frac.device))) + 1
+            evals = evecs = None
+            if self.basis_solver == "subspace":
+                evals, evecs = leading_eigh(m2, rounded(512))
+                if float(evals.clamp_min(0.0).sum()) < explained * trace:
+                    evals = evecs = None  # the leading 512 fall short: full solve
+            if evals is None:
+                evals, evecs = fp64_eigh(m2)
+            evals = evals.clamp_min(0.0)
+            frac = torch.cumsum(evals.flip(0), 0) / trace
+            k = rounded(int(torch.searchsorted(frac, torch.tensor([explained], dtype=frac.dtype,
+                                                                   device=frac.device))) + 1)
         else:
-            k = int(components)
-        k = min(max(-(-k // 64) * 64, k_min), d)
+            k = rounded(int(components))
+            evals, evecs = leading_eigh(m2, k) if self.basis_solver == "subspace" else fp64_eigh(m2)
+            evals = evals.clamp_min(0.0)
         self.search_components = k
-        self.explained = float(evals[-k:].sum() / evals.sum())
+        self.explained = float(evals[-k:].sum() / trace)
         basis = evecs[:, -k:]  # (d, k) float64
         if self.rotate_basis:
             # a fixed random rotation of the subspace: the dot products are
