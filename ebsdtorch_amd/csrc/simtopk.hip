@@ -202,11 +202,12 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 //
 //   block   : 256 threads = 4 waves, one workgroup per CU (256 persistent
 //             workgroups = 8 XCDs x 32); block tile 192 dictionary rows x 256
-//             queries; wave (wd = w % 2, wq = w / 2) owns dictionary rows
-//             [96 wd, 96 wd + 96) and queries [128 wq, 128 wq + 128): 3 x 4
-//             MFMA 32x32 tiles (192 accumulator registers, leaving the
-//             register allocator a quarter of the AGPR file), so every
-//             operand byte read from LDS feeds three or four MFMAs
+//             queries; wave w owns queries [64 w, 64 w + 64) against all 192
+//             dictionary rows of the tile: 6 x 2 MFMA 32x32 tiles (192
+//             accumulator registers, leaving the register allocator a quarter
+//             of the AGPR file). Every score of a query is made in one wave,
+//             so each query keeps a single list whose own K-th best is the
+//             exact bound of what may still enter it
 //   stages  : 64 bytes of K for all 448 rows (28 KiB) land in LDS straight
 //             from global memory (global_load_lds_dwordx4, 7 per wave per
 //             stage, plus one 4-byte load of the next tile's dictionary
@@ -214,7 +215,12 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 //             stage g - 1, so three stages are in flight while one is read.
 //             Issuing is branch-free: past the last stage the loads fetch
 //             clamped rows into the free buffer, so every stage waits for the
-//             same count and the MFMAs can be scheduled across the issue
+//             same count and the MFMAs can be scheduled across the issue.
+//             Rows of 256 bytes (4 stages) keep the work unit's 256 query
+//             rows resident in LDS instead (loaded once, 64 KiB), and the
+//             ring carries the 192 dictionary rows only (12 KiB, 3 loads per
+//             wave and stage plus the scales): the query reload was a
+//             quarter of the kernel's time at that width
 //   barrier : one per stage, a bare s_barrier: each wave's own vmcnt wait
 //             covers the direct loads it issued, so no full memory fence
 //   LDS     : rows are 64 bytes, unpadded; 16-byte chunk c of row r sits at
@@ -227,34 +233,43 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 //             op_sel), so scores come out scaled with no epilogue multiply
 //   top-k   : the two lane halves of a 32x32 tile hold the same 32 query
 //             columns but different rows; lane half h keeps the lists of
-//             column blocks 2 h and 2 h + 1, two per lane. A tile's scores
-//             are screened in place (per lane and 16 rows, the maximum
-//             against the threshold of the score's column); a 32-row block
-//             that some lane may take is written to the wave's LDS scratch
-//             in the rows of the lanes that own its columns, and each owner
-//             offers its marked scores in ascending row order. At the end of
-//             a work unit the 2 lists of each query (one per dictionary
-//             wave) are merged through LDS into one partial list.
+//             column block h, one per lane. A tile's scores are screened in
+//             place (per lane and 16 rows, the maximum against the K-th best
+//             of the score's column); a 32-row block that some lane may take
+//             is written to the wave's LDS scratch in the rows of the lanes
+//             that own its columns, and each owner offers its marked scores
+//             in ascending row order. At the end of a work unit each lane
+//             writes its list as the partial list of its query.
 constexpr int BT_D = 192;                                 // dictionary rows per tile
 constexpr int BT_Q = 256;                                 // queries per tile
-constexpr int BT_WD = 96;                                 // dictionary rows per wave
+constexpr int BT_WQ = 64;                                 // queries per wave
 constexpr int BT_KB = 64;                                 // bytes of K per row per stage
 constexpr int BT_STAGE = (BT_D + BT_Q) * BT_KB;           // 28 KiB
 constexpr int BT_NBUF = 4;
 constexpr int BT_SCALE_OFF = BT_NBUF * BT_STAGE;          // two tiles' dictionary scales
-constexpr int BT_THR_OFF = BT_SCALE_OFF + 2 * 256 * 4;    // list bounds [tile parity][wd][2][query]
 // Scratch row of one lane in the fold: the 32 scores of a block for the query
 // column whose list the lane owns, in row order (36 floats, so the 16-byte
 // pieces written by 16 consecutive lanes start in 16 different bank groups).
 constexpr int BT_SCR_ROW = 36;
-constexpr int BT_SCR_OFF = BT_THR_OFF + 2 * 2 * 2 * BT_Q * 4; // fold scratch, 9 KiB per wave
+constexpr int BT_SCR_OFF = BT_SCALE_OFF + 2 * 256 * 4;    // fold scratch, 9 KiB per wave
 constexpr int BT_LDS = BT_SCR_OFF + 4 * 64 * BT_SCR_ROW * 4;
 constexpr int BT_THREADS = 256;
 // direct loads per wave per stage: 1-KiB data loads plus one scale load
 constexpr int BT_LOADS = BT_STAGE / 1024 / (BT_THREADS / 64) + 1;
 static_assert(BT_LOADS == 8, "the stage waits of sim_topk_bt_kernel count 8 loads per wave");
+// Resident-query mode, for rows of exactly 4 stages (256 bytes): the 256
+// query rows of a work unit (64 KiB) are loaded into LDS once, and the stage
+// ring carries only the 192 dictionary rows (12 KiB per stage). Ring and query
+// block together take the 112 KiB of the four full stages.
+constexpr int BT_RQ_ROW_BYTES = 4 * BT_KB;
+constexpr int BT_RQ_STAGE = BT_D * BT_KB;                    // 12 KiB
+constexpr int BT_RQ_QOFF = BT_NBUF * BT_RQ_STAGE;            // query block
+constexpr int BT_RQ_QSTAGE = BT_Q * BT_KB;                   // 16 KiB of it per stage
+constexpr int BT_RQ_LOADS = BT_RQ_STAGE / 1024 / (BT_THREADS / 64) + 1;
+static_assert(BT_RQ_LOADS == 4, "the stage waits of the resident-query mode count 4 loads per wave");
+static_assert(BT_RQ_QOFF + 4 * BT_RQ_QSTAGE == BT_SCALE_OFF, "resident queries fill the full stage ring");
 static_assert(BT_LDS <= 160 * 1024, "LDS budget");
-static_assert(BT_Q * 2 * EBSD_MAX_TOPK * 8 <= BT_SCALE_OFF, "the list merge reuses the stage ring");
+static_assert(BT_Q == 4 * BT_WQ && BT_D == 6 * 32, "four waves of 64 queries, six 32-row blocks");
 constexpr int BT_SLOTS_PER_XCD = 32;
 constexpr int BT_MIN_STAGES = 4;  // the pipeline and the scale slots assume at least 4 stages per tile
 
@@ -293,50 +308,58 @@ __device__ __forceinline__ void bt_wait_barrier() {
   asm volatile("" ::: "memory");
 }
 
-// MFMA operands of one stage for one wave: A = its 96 dictionary rows (three
-// 32-row blocks), B = its 128 queries (four blocks), 32 bytes of K per lane
+// MFMA operands of one stage for one wave: A = all 192 dictionary rows (six
+// 32-row blocks), B = its 64 queries (two blocks), 32 bytes of K per lane
 // and block.
 struct BtFrag {
-  intx8 a[3];
-  intx8 b[4];
+  intx8 a[6];
+  intx8 b[2];
 };
 
 // Lane half h reads, per 32-row block, the 32 bytes of the 64-byte stage that
 // its MFMAs consume in the K order of lane_k_off: FP8 one 32x32x64 step (bytes
 // 32 h .. 32 h + 31 = chunks 2 h, 2 h + 1), BF16/FP16 two 32x32x16 steps
-// (chunk h, then chunk 2 + h). Reads are issued in first-use order.
+// (chunk h, then chunk 2 + h). Reads are issued in first-use order. abuf:
+// the stage's dictionary rows, bbuf: its query rows (the 16 bytes of row r,
+// chunk c at bt_swz(r, c) from either base).
 template <int DT>
-__device__ __forceinline__ void bt_read(const unsigned char* __restrict__ buf, int wd, int wq, int lane, BtFrag& f) {
-  const int r32 = lane & 31, h = lane >> 5;
+__device__ __forceinline__ intx8 bt_rd(const unsigned char* __restrict__ buf, int row, int lane) {
+  const int h = lane >> 5;
   const int c0 = DT == EBSD_DTYPE_FP8_E4M3 ? 2 * h : h;
   const int c1 = DT == EBSD_DTYPE_FP8_E4M3 ? 2 * h + 1 : 2 + h;
-  auto rd = [&](int row) {
-    const intx4 x0 = *reinterpret_cast<const intx4*>(buf + bt_swz(row, c0));
-    const intx4 x1 = *reinterpret_cast<const intx4*>(buf + bt_swz(row, c1));
-    return intx8{x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-  };
-  f.a[0] = rd(BT_WD * wd + r32);
-#pragma unroll
-  for (int j = 0; j < 4; ++j) f.b[j] = rd(BT_D + 128 * wq + 32 * j + r32);
-#pragma unroll
-  for (int i = 1; i < 3; ++i) f.a[i] = rd(BT_WD * wd + 32 * i + r32);
+  const intx4 x0 = *reinterpret_cast<const intx4*>(buf + bt_swz(row, c0));
+  const intx4 x1 = *reinterpret_cast<const intx4*>(buf + bt_swz(row, c1));
+  return intx8{x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
 }
 
-// MFMAs IJ .. END - 1 (I = IJ / 4 dictionary block, J = IJ % 4 query block)
+template <int DT>
+__device__ __forceinline__ void bt_read(const unsigned char* __restrict__ abuf,
+                                        const unsigned char* __restrict__ bbuf, int wq, int lane, BtFrag& f) {
+  const int r32 = lane & 31;
+  f.a[0] = bt_rd<DT>(abuf, r32, lane);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) f.b[j] = bt_rd<DT>(bbuf, BT_WQ * wq + 32 * j + r32, lane);
+#pragma unroll
+  for (int i = 1; i < 6; ++i) f.a[i] = bt_rd<DT>(abuf, 32 * i + r32, lane);
+}
+
+// MFMAs IJ .. END - 1 (I = IJ / 2 dictionary block, J = IJ % 2 query block)
 // of K step S of one stage (FP8: one step of 64, BF16/FP16: two of 16).
-// FP8: sa / sb hold the E8M0 exponents of the lane's dictionary / query row
-// in byte i / j. Z: the first step of a tile, which starts the accumulators
-// from zero (the MFMA's C operand is the constant 0).
+// FP8: byte i % 4 of sa[i / 4] / byte j of sb hold the E8M0 exponents of the
+// lane's dictionary / query row in block i / j. Z: the first step of a tile,
+// which starts the accumulators from zero (the MFMA's C operand is the
+// constant 0).
 template <int DT, int S, int IJ, int END, bool Z>
-__device__ __forceinline__ void bt_mfma_step(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4]) {
+__device__ __forceinline__ void bt_mfma_step(const intx8 (&a)[6], const intx8 (&b)[2], const int (&sa)[2], int sb,
+                                             floatx16 (&acc)[6][2]) {
   if constexpr (IJ < END) {
-    constexpr int I = IJ / 4, J = IJ % 4;
+    constexpr int I = IJ / 2, J = IJ % 2;
     const floatx16 c = Z ? floatx16{} : acc[I][J];
     if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
-      acc[I][J] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(f.a[I], f.b[J], c, 0, 0, I, sa, J, sb);
+      acc[I][J] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[I], b[J], c, 0, 0, I & 3, sa[I >> 2], J, sb);
     } else {
-      const intx4 a4 = {f.a[I][4 * S], f.a[I][4 * S + 1], f.a[I][4 * S + 2], f.a[I][4 * S + 3]};
-      const intx4 b4 = {f.b[J][4 * S], f.b[J][4 * S + 1], f.b[J][4 * S + 2], f.b[J][4 * S + 3]};
+      const intx4 a4 = {a[I][4 * S], a[I][4 * S + 1], a[I][4 * S + 2], a[I][4 * S + 3]};
+      const intx4 b4 = {b[J][4 * S], b[J][4 * S + 1], b[J][4 * S + 2], b[J][4 * S + 3]};
       if constexpr (DT == EBSD_DTYPE_BF16)
         acc[I][J] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a4),
                                                             __builtin_bit_cast(bf16x8, b4), c, 0, 0, 0);
@@ -344,21 +367,25 @@ __device__ __forceinline__ void bt_mfma_step(const BtFrag& f, int sa, int sb, fl
         acc[I][J] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(halfx8, a4),
                                                            __builtin_bit_cast(halfx8, b4), c, 0, 0, 0);
     }
-    bt_mfma_step<DT, S, IJ + 1, END, Z>(f, sa, sb, acc);
+    bt_mfma_step<DT, S, IJ + 1, END, Z>(a, b, sa, sb, acc);
   }
 }
 
-// All MFMAs of one stage, with hook(k) called after the k-th of the first 8
-// (k = 0 .. 7): the loads of a later stage go out in the shadow of the MFMAs.
-// Z: the tile's first stage (accumulators start from zero).
-template <int DT, bool Z, int IJ = 0, typename Hook>
-__device__ __forceinline__ void bt_mfma_hooked(const BtFrag& f, int sa, int sb, floatx16 (&acc)[3][4], Hook&& hook) {
-  if constexpr (IJ < 12) {
-    bt_mfma_step<DT, 0, IJ, IJ + 1, Z>(f, sa, sb, acc);
-    if constexpr (IJ < 8) hook(IJ);
-    bt_mfma_hooked<DT, Z, IJ + 1>(f, sa, sb, acc, hook);
-  } else if constexpr (DT != EBSD_DTYPE_FP8_E4M3) {
-    bt_mfma_step<DT, 1, 0, 12, false>(f, sa, sb, acc);
+// All MFMAs of one stage (K steps S < NS, MFMAs IJ < 12 each: FP8 one step,
+// BF16 / FP16 two), with hook(integral_constant<12 S + IJ>) called after each:
+// the loads of a later stage go out in the shadow of the MFMAs, and the
+// resident-query mode reads the next stage's dictionary operands into the
+// registers an MFMA pair has finished with. Z: the tile's first stage
+// (accumulators start from zero).
+template <int DT, bool Z, int K = 0, typename Hook>
+__device__ __forceinline__ void bt_mfma_hooked(const intx8 (&a)[6], const intx8 (&b)[2], const int (&sa)[2], int sb,
+                                               floatx16 (&acc)[6][2], Hook&& hook) {
+  constexpr int NS = DT == EBSD_DTYPE_FP8_E4M3 ? 1 : 2;
+  if constexpr (K < 12 * NS) {
+    constexpr int S = K / 12, IJ = K % 12;
+    bt_mfma_step<DT, S, IJ, IJ + 1, Z && S == 0>(a, b, sa, sb, acc);
+    hook(std::integral_constant<int, K>{});
+    bt_mfma_hooked<DT, Z, K + 1>(a, b, sa, sb, acc, hook);
   }
 }
 
@@ -396,23 +423,26 @@ __device__ __forceinline__ float bt_max16(const floatx16& a) {
 // offers that arrive in ascending id order: an equal score then never
 // outranks a listed one (whose id is smaller), so one comparison per step
 // gives the order of topk_better.
+//
+// The new entry goes right after the last entry it does not beat: with
+// c_j = v > tv[j] (monotone in j), entry j becomes tv[j - 1] where c_{j-1},
+// else v where c_j, else stays, i.e. the median of (tv[j - 1], v, tv[j]) for
+// the value (one v_med3_f32; the scores are finite) and two selects for the
+// index. An offer that does not beat tv[K - 1] (all c_j false) leaves the
+// list as it is, so the update runs without a branch.
 template <int K>
 __device__ __forceinline__ void topk_offer_ascending(float (&tv)[K], int (&ti)[K], float v, int id) {
-  if (!(v > tv[K - 1])) return;
-  tv[K - 1] = v;
-  ti[K - 1] = id;
+  bool cj = v > tv[K - 1];
 #pragma unroll
   for (int j = K - 1; j > 0; --j) {
-    const bool b = tv[j] > tv[j - 1];
-    const float va = b ? tv[j] : tv[j - 1];
-    const float vb = b ? tv[j - 1] : tv[j];
-    const int ia = b ? ti[j] : ti[j - 1];
-    const int ib = b ? ti[j - 1] : ti[j];
-    tv[j - 1] = va;
-    tv[j] = vb;
-    ti[j - 1] = ia;
-    ti[j] = ib;
+    const bool cp = v > tv[j - 1];
+    const int t = cj ? id : ti[j];
+    ti[j] = cp ? ti[j - 1] : t;
+    tv[j] = __builtin_amdgcn_fmed3f(tv[j - 1], v, tv[j]);
+    cj = cp;
   }
+  ti[0] = cj ? id : ti[0];
+  tv[0] = cj ? v : tv[0];
 }
 
 // Bit j = 4 q + rr of a 16-bit mask to bit 8 q + rr (q, rr < 4).
@@ -421,111 +451,101 @@ __device__ __forceinline__ unsigned bt_spread_rows(unsigned p) {
   return (y | (y << 4)) & 0x0f0f0f0fu;
 }
 
-// Fold a finished 192 x 256 tile into the lane's two lists. Lane (h, c)
+// Fold a finished 192 x 64 wave tile into the lane's list. Lane (h, c)
 // (h = lane / 32, c = lane % 32) holds, of every 32 x 32 tile, column c and
-// rows 8 q + 4 h + rr (accumulator register 4 q + rr); its list s belongs to
-// column c of column block s + 2 h. dbase: index of the wave's first
-// dictionary row; n_valid: how many of its 96 rows exist; thr[s]: a lower
-// bound of the final K-th best score of list s's query; scratch: the wave's
-// 64 scratch rows. Offers reach each list in ascending dictionary index.
+// rows 8 q + 4 h + rr (accumulator register 4 q + rr); its list belongs to
+// column c of column block h. dbase: index of the tile's first dictionary
+// row; n_valid: how many of its 192 rows exist; scratch: the wave's 64
+// scratch rows. Offers reach the list in ascending dictionary index.
 template <int K>
-__device__ __forceinline__ void bt_fold(const floatx16 (&acc)[3][4], float (&tv)[2][K], int (&ti)[2][K],
-                                        const float (&thr)[2], int dbase, int n_valid, float* scratch, int lane) {
+__device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)[K], int (&ti)[K], int dbase,
+                                        int n_valid, float* scratch, int lane) {
   const int h = lane >> 5, c = lane & 31;
-  // the thresholds of the two columns whose scores the lane holds: column c
-  // of block s (list s of lane c) and of block s + 2 (list s of lane 32 + c)
-  float ta[2], tb[2];
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(thr[s]), __float_as_uint(thr[s]), false, false);
-    ta[s] = __uint_as_float(t[0]);
-    tb[s] = __uint_as_float(t[1]);
-  }
-  // Screen every 32-row block first, with no branch in between: bit
-  // 2 (3 s + i) (resp. + 1) is set when some lane may take a score of column
-  // block s (resp. s + 2) in row block i.
+  // the K-th best scores of the two columns whose scores the lane holds:
+  // column c of block 0 (the list of lane c) and of block 1 (lane 32 + c)
+  const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(tv[K - 1]), __float_as_uint(tv[K - 1]), false,
+                                                  false);
+  const float ta = __uint_as_float(t[0]);
+  const float tb = __uint_as_float(t[1]);
+  // Screen every 32-row block first, with no branch in between: bit 2 i
+  // (resp. 2 i + 1) is set when some lane may take a score of column block 0
+  // (resp. 1) in row block i.
   unsigned hits = 0;
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const float ma = bt_max16(acc[i][s]);
-      const float mb = bt_max16(acc[i][s + 2]);
-      if (__builtin_amdgcn_ballot_w64(!(ma < ta[s])) != 0) hits |= 1u << (2 * (3 * s + i));
-      if (__builtin_amdgcn_ballot_w64(!(mb < tb[s])) != 0) hits |= 2u << (2 * (3 * s + i));
-    }
+  for (int i = 0; i < 6; ++i) {
+    const float ma = bt_max16(acc[i][0]);
+    const float mb = bt_max16(acc[i][1]);
+    if (__builtin_amdgcn_ballot_w64(!(ma < ta)) != 0) hits |= 1u << (2 * i);
+    if (__builtin_amdgcn_ballot_w64(!(mb < tb)) != 0) hits |= 2u << (2 * i);
   }
   if (hits == 0) return;
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  for (int i = 0; i < 6; ++i) {
+    const unsigned hb = (hits >> (2 * i)) & 3u;
+    if (hb == 0) continue;
+    // Some lane may take a score: write the scores of the column block (or
+    // blocks) concerned into the scratch rows of the lanes that own their
+    // columns (row 8 q + 4 h + rr at float 8 q + 4 h + rr), mark the ones
+    // that pass the owner's bound, and let every owner walk its marked rows
+    // in ascending order (a divergent loop of max-popcount trips).
+    // fail bit j = 4 q + rr: the score of row 8 q + 4 h + rr is below the
+    // owner's bound, i.e. the sign of score - bound (+0 for equal finite
+    // values), shifted in from j = 15 down to j = 0; a block that is not
+    // staged fails everywhere
+    unsigned fail_a = 0xffffu, fail_b = 0xffffu;
+    if (hb & 1u) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const unsigned hb = (hits >> (2 * (3 * s + i))) & 3u;
-      if (hb == 0) continue;
-      // Some lane may take a score: write the scores of the column half (or
-      // halves) concerned into the scratch rows of the lanes that own their
-      // columns (row 8 q + 4 h + rr at float 8 q + 4 h + rr), mark the ones
-      // that pass the owner's threshold, and let every owner walk its marked
-      // rows in ascending order (a divergent loop of max-popcount trips).
-      // fail bit j = 4 q + rr: the score of row 8 q + 4 h + rr is below the
-      // owner's threshold, i.e. the sign of score - threshold (+0 for equal
-      // finite values), shifted in from j = 15 down to j = 0; a half that is
-      // not staged fails everywhere
-      unsigned fail_a = 0xffffu, fail_b = 0xffffu;
-      if (hb & 1u) {
+      for (int q = 3; q >= 0; --q) {
+        floatx4 va;
 #pragma unroll
-        for (int q = 3; q >= 0; --q) {
-          floatx4 va;
+        for (int rr = 0; rr < 4; ++rr) va[rr] = bt_acc(acc[i][0][4 * q + rr]);
+        *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * c + 8 * q + 4 * h) = va;
 #pragma unroll
-          for (int rr = 0; rr < 4; ++rr) va[rr] = bt_acc(acc[i][s][4 * q + rr]);
-          *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * c + 8 * q + 4 * h) = va;
-#pragma unroll
-          for (int rr = 3; rr >= 0; --rr)
-            fail_a = __builtin_amdgcn_alignbit(fail_a, __float_as_uint(va[rr] - ta[s]), 31);
-        }
+        for (int rr = 3; rr >= 0; --rr) fail_a = __builtin_amdgcn_alignbit(fail_a, __float_as_uint(va[rr] - ta), 31);
       }
-      if (hb & 2u) {
-#pragma unroll
-        for (int q = 3; q >= 0; --q) {
-          floatx4 vb;
-#pragma unroll
-          for (int rr = 0; rr < 4; ++rr) vb[rr] = bt_acc(acc[i][s + 2][4 * q + rr]);
-          *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * (32 + c) + 8 * q + 4 * h) = vb;
-#pragma unroll
-          for (int rr = 3; rr >= 0; --rr)
-            fail_b = __builtin_amdgcn_alignbit(fail_b, __float_as_uint(vb[rr] - tb[s]), 31);
-        }
-      }
-      const unsigned bits_a = bt_spread_rows(~fail_a & 0xffffu) << (4 * h);
-      const unsigned bits_b = bt_spread_rows(~fail_b & 0xffffu) << (4 * h);
-      // the marks of the owned column from the partner lane (the other half)
-      const unsigned other = h ? bits_a : bits_b;
-      const auto o = __builtin_amdgcn_permlane32_swap(other, other, false, false);
-      unsigned m = (h ? bits_b : bits_a) | (h ? o[0] : o[1]);
-      // The partner's stores precede these loads in the wave's LDS order; the
-      // clobbers keep the compiler from moving memory accesses across.
-      asm volatile("" ::: "memory");
-      // the score of the next marked row is read ahead of the current row's
-      // insertion (for an empty rest, row 31's: a harmless in-bounds read)
-      int rho = __builtin_ctz(m | 0x80000000u);
-      float v = scratch[BT_SCR_ROW * lane + rho];
-      while (m != 0) {
-        m &= m - 1;
-        const int rho_next = __builtin_ctz(m | 0x80000000u);
-        const float v_next = scratch[BT_SCR_ROW * lane + rho_next];
-        const int row = 32 * i + rho;
-        if (row < n_valid) topk_offer_ascending<K>(tv[s], ti[s], v, dbase + row);
-        rho = rho_next;
-        v = v_next;
-      }
-      asm volatile("" ::: "memory");
     }
+    if (hb & 2u) {
+#pragma unroll
+      for (int q = 3; q >= 0; --q) {
+        floatx4 vb;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) vb[rr] = bt_acc(acc[i][1][4 * q + rr]);
+        *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * (32 + c) + 8 * q + 4 * h) = vb;
+#pragma unroll
+        for (int rr = 3; rr >= 0; --rr) fail_b = __builtin_amdgcn_alignbit(fail_b, __float_as_uint(vb[rr] - tb), 31);
+      }
+    }
+    const unsigned bits_a = bt_spread_rows(~fail_a & 0xffffu) << (4 * h);
+    const unsigned bits_b = bt_spread_rows(~fail_b & 0xffffu) << (4 * h);
+    // the marks of the owned column from the partner lane (the other half)
+    const unsigned other = h ? bits_a : bits_b;
+    const auto o = __builtin_amdgcn_permlane32_swap(other, other, false, false);
+    unsigned m = (h ? bits_b : bits_a) | (h ? o[0] : o[1]);
+    // The partner's stores precede these loads in the wave's LDS order; the
+    // clobbers keep the compiler from moving memory accesses across.
+    asm volatile("" ::: "memory");
+    // the score of the next marked row is read ahead of the current row's
+    // insertion (for an empty rest, row 31's: a harmless in-bounds read)
+    int rho = __builtin_ctz(m | 0x80000000u);
+    float v = scratch[BT_SCR_ROW * lane + rho];
+    while (m != 0) {
+      m &= m - 1;
+      const int rho_next = __builtin_ctz(m | 0x80000000u);
+      const float v_next = scratch[BT_SCR_ROW * lane + rho_next];
+      const int row = 32 * i + rho;
+      // a row past the dictionary's end is offered as -inf (no change)
+      topk_offer_ascending<K>(tv, ti, row < n_valid ? v : -INFINITY, dbase + row);
+      rho = rho_next;
+      v = v_next;
+    }
+    asm volatile("" ::: "memory");
   }
 }
 
 // Requires N >= BT_D and at least BT_MIN_STAGES stages (ld_bytes >= 256); see
-// plan_sim. FP8 scales are used through their exponents (powers of two).
-template <int DT, int K>
+// plan_sim. FP8 scales are used through their exponents (powers of two). RQ:
+// resident-query mode (ld_bytes == BT_RQ_ROW_BYTES).
+template <int DT, int K, bool RQ>
 __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     const unsigned char* __restrict__ Q, const float* __restrict__ q_scale, int M,
     const unsigned char* __restrict__ Dm, const float* __restrict__ d_scale, int N, int ld_bytes,
@@ -534,7 +554,7 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wd = wave & 1, wq = wave >> 1;
+  const int wq = wave;
   const int r32 = lane & 31, h = lane >> 5;
   // 32 slots per XCD (one workgroup per CU); the mapping is slot_work's
   const SlotWork wk = slot_work(sc, blockIdx.x, (N + BT_D - 1) / BT_D);
@@ -552,7 +572,6 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
   const int lrow = lane >> 2;
   const int lchunk = (lane & 3) ^ ((lane >> 4) & 3);
   const float* sscale = reinterpret_cast<const float*>(smem + BT_SCALE_OFF);
-  float* lthr = reinterpret_cast<float*>(smem + BT_THR_OFF);
   float* scratch = reinterpret_cast<float*>(smem + BT_SCR_OFF) + wave * 64 * BT_SCR_ROW;
 
   for (int grp = wk.g_first; grp < sc.n_qgroups; grp += wk.g_step) {
@@ -570,13 +589,12 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     int sb = 0;
     if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) sb |= e8m0_of(q_scale[min(q0 + 128 * wq + 32 * j + r32, M - 1)]) << (8 * j);
+      for (int j = 0; j < 2; ++j) sb |= e8m0_of(q_scale[min(q0 + BT_WQ * wq + 32 * j + r32, M - 1)]) << (8 * j);
     }
-    float tv[2][K];
-    int ti[2][K];
-    topk_init<K>(tv[0], ti[0]);
-    topk_init<K>(tv[1], ti[1]);
-    floatx16 acc[3][4];  // written from zero by the first stage of every tile
+    float tv[K];
+    int ti[K];
+    topk_init<K>(tv, ti);
+    floatx16 acc[6][2];  // written from zero by the first stage of every tile
 
     // Loads of stage g, in 8 parts: k < 3 dictionary rows 16 (wave + 4 k) ..,
     // k = 3 .. 6 query rows 16 (wave + 4 (k - 3)) .., k = 7 the dictionary
@@ -593,17 +611,21 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
         return Dm + (size_t)min(t * BT_D, N - 1) * ld_bytes + 4 * lane;
     };
     auto scale_m0 = [&](int t) { return lds0 + BT_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
+    // RQ: the loads of stage g are its three dictionary parts and the scales
+    // (k = 0 .. 3); the query rows are loaded once per work unit (qload)
+    constexpr int LOADS = RQ ? BT_RQ_LOADS : BT_LOADS;
+    constexpr int STAGE = RQ ? BT_RQ_STAGE : BT_STAGE;
     auto issue_part = [&](int g, int k) {
       const int tl = g / nk;
       const int kt = g - tl * nk;
       const int t = t_begin + tl;
-      const unsigned mb = lds0 + (g % BT_NBUF) * BT_STAGE;
+      const unsigned mb = lds0 + (g % BT_NBUF) * STAGE;
       if (k < 3) {
         const int L = wave + 4 * k;
         const unsigned char* src =
             Dm + (size_t)min(t * BT_D + 16 * L + lrow, N - 1) * ld_bytes + lchunk * 16 + kt * BT_KB;
         bt_dma16(src, mb + (wave_s + 4 * k) * 1024);
-      } else if (k < 7) {
+      } else if (!RQ && k < 7) {
         bt_dma16(qsrc[k - 3] + kt * BT_KB, mb + (12 + wave_s + 4 * (k - 3)) * 1024);
       } else {
         bt_dma4(scale_src(t + 1), scale_m0(t + 1));
@@ -611,120 +633,132 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     };
     auto issue = [&](int g) {
 #pragma unroll
-      for (int k = 0; k < BT_LOADS; ++k) issue_part(g, k);
+      for (int k = 0; k < LOADS; ++k) issue_part(g, k);
     };
-    // E8M0 exponents of the lane's dictionary rows of tile t (byte i: block
-    // i). The result passes through a move where it is read, so its LDS wait
-    // is settled here and not at the MFMAs of the next stage.
-    auto dict_scales = [&](int t) {
-      int s = 0;
-      if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
-        const float* sl = sscale + (t & 1) * 256 + BT_WD * wd + r32;
+    // RQ: the work unit's query rows, stage kt's 64 bytes of all 256 rows at
+    // BT_RQ_QOFF + kt * BT_RQ_QSTAGE in the stages' row layout (16 loads)
+    auto qload = [&]() {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) s |= e8m0_of(sl[32 * i]) << (8 * i);
-        asm volatile("v_mov_b32 %0, %1" : "=v"(s) : "v"(s));
+      for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          bt_dma16(qsrc[j] + kt * BT_KB, lds0 + BT_RQ_QOFF + kt * BT_RQ_QSTAGE + (wave_s + 4 * j) * 1024);
+    };
+    // E8M0 exponents of the lane's dictionary rows of tile t (byte i % 4 of
+    // s[i / 4]: block i). The results pass through a move where they are
+    // read, so their LDS wait is settled here and not at the MFMAs of the
+    // next stage.
+    auto dict_scales = [&](int t, int (&s)[2]) {
+      s[0] = s[1] = 0;
+      if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+        const float* sl = sscale + (t & 1) * 256 + r32;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i >> 2] |= e8m0_of(sl[32 * i]) << (8 * (i & 3));
+        asm volatile("v_mov_b32 %0, %1" : "=v"(s[0]) : "v"(s[0]));
+        asm volatile("v_mov_b32 %0, %1" : "=v"(s[1]) : "v"(s[1]));
       }
-      return s;
     };
 
-    // Stage g is complete once the two batches issued after it are the only
-    // ones in flight (loads complete in issue order); the barrier then also
-    // means every wave is done reading the stages before g.
-    // prologue: the first tile's scales and stages 0 .. 2 in flight
-    int sa = 0;
-    if (G > 0) {
-      bt_dma4(scale_src(t_begin), scale_m0(t_begin));
-      for (int p = 0; p < BT_NBUF - 1; ++p) issue(p);
-      bt_wait_barrier<2 * BT_LOADS>();
-      sa = dict_scales(t_begin);
-      __builtin_amdgcn_s_waitcnt(0xC07F);
-    }
+    int sa[2] = {0, 0};
     const int n_t = G / max(nk, 1);
-    for (int tl = 0; tl < n_t; ++tl) {
-      // stage g of the tile; first: the tile's first stage
-      auto stage = [&](int g, auto first) {
-        bt_wait_barrier<2 * BT_LOADS>();
-        BtFrag f;
-        bt_read<DT>(smem + (g % BT_NBUF) * BT_STAGE, wd, wq, lane, f);
-        __builtin_amdgcn_sched_barrier(0);
-        // the MFMAs start as their operands come in; the loads of stage
-        // g + 3 (into the buffer of stage g - 1) go out between them
-        bt_mfma_hooked<DT, decltype(first)::value>(f, sa, sb, acc,
-                                                   [&](int k) { issue_part(g + BT_NBUF - 1, k); });
-      };
-      stage(tl * nk, std::true_type{});
-      for (int kt = 1; kt < nk; ++kt) stage(tl * nk + kt, std::false_type{});
-      // A score below the K-th best over the query's 2 lists (one per
-      // dictionary wave, disjoint rows) cannot reach its final top-K. Lower
-      // bounds of that: either list's K-th best, and the smaller of the two
-      // lists' ((K + 1) / 2)-th bests (together at least K scores at least as
-      // good). The other wave's values are those after the previous tile
-      // (they only rise); offers are filtered with the largest bound.
-      float thr[2];
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int ql = 128 * wq + 32 * (s + 2 * h) + r32;
-        const float own = tv[s][K - 1];
-        const float own_mid = tv[s][(K + 1) / 2 - 1];
-        float th = own;
-        if (tl > 0) {
-          const float* o = lthr + ((((tl - 1) & 1) * 2 + (wd ^ 1)) * 2) * BT_Q + ql;
-          th = fmaxf(th, fmaxf(o[0], fminf(own_mid, o[BT_Q])));
-        }
-        thr[s] = th;
-        float* w = lthr + (((tl & 1) * 2 + wd) * 2) * BT_Q + ql;
-        w[0] = own;
-        w[BT_Q] = own_mid;
-      }
-      const int d0w = (t_begin + tl) * BT_D + BT_WD * wd;
-      bt_fold<K>(acc, tv, ti, thr, index_base + d0w, N - d0w, scratch, lane);
+    auto fold = [&](int tl) {
+      const int d0 = (t_begin + tl) * BT_D;
+      bt_fold<K>(acc, tv, ti, index_base + d0, N - d0, scratch, lane);
       // the next tile's scales came with this tile's stages
-      if (tl + 1 < n_t) sa = dict_scales(t_begin + tl + 1);
+      if (tl + 1 < n_t) dict_scales(t_begin + tl + 1, sa);
       // no LDS operation of the fold stays in flight into the next tile's
       // stages (lgkmcnt(0); vmcnt and expcnt unconstrained), so the waits
       // there only count the stage's own reads
       __builtin_amdgcn_s_waitcnt(0xC07F);
+    };
+    if constexpr (!RQ) {
+      // Stage g is complete once the two batches issued after it are the
+      // only ones in flight (loads complete in issue order); the barrier then
+      // also means every wave is done reading the stages before g.
+      // prologue: the first tile's scales and stages 0 .. 2 in flight
+      if (G > 0) {
+        bt_dma4(scale_src(t_begin), scale_m0(t_begin));
+        for (int p = 0; p < BT_NBUF - 1; ++p) issue(p);
+        bt_wait_barrier<2 * LOADS>();
+        dict_scales(t_begin, sa);
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+      }
+      for (int tl = 0; tl < n_t; ++tl) {
+        // stage g of the tile; first: the tile's first stage
+        auto stage = [&](int g, auto first) {
+          bt_wait_barrier<2 * LOADS>();
+          BtFrag f;
+          const unsigned char* abuf = smem + (g % BT_NBUF) * STAGE;
+          bt_read<DT>(abuf, abuf + BT_D * BT_KB, wq, lane, f);
+          __builtin_amdgcn_sched_barrier(0);
+          // the MFMAs start as their operands come in; the loads of stage
+          // g + 3 (into the buffer of stage g - 1) go out between them
+          bt_mfma_hooked<DT, decltype(first)::value>(f.a, f.b, sa, sb, acc, [&](auto k) {
+            if constexpr (decltype(k)::value < LOADS) issue_part(g + BT_NBUF - 1, decltype(k)::value);
+          });
+        };
+        stage(tl * nk, std::true_type{});
+        for (int kt = 1; kt < nk; ++kt) stage(tl * nk + kt, std::false_type{});
+        fold(tl);
+      }
+    } else {
+      // Resident queries (nk == 4): the wave's query operands of all four
+      // stages stay in registers for the whole unit (bres, 64 VGPRs), and the
+      // dictionary operands of stage g + 1 are read during stage g's MFMAs,
+      // block i into a[i] as soon as its two MFMAs have been issued. Stage
+      // g's barrier therefore means: stage g + 1 has landed, and every wave
+      // has read stage g, whose buffer then takes stage g + 4 (the lead stays
+      // three stages: two batches in flight at every wait).
+      // prologue: the queries, the first tile's scales and stages 0 .. 3
+      intx8 bres[4][2];
+      intx8 a[6];
+      if (G > 0) {
+        bt_dma4(scale_src(t_begin), scale_m0(t_begin));
+        qload();
+        for (int p = 0; p < BT_NBUF; ++p) issue(p);
+        bt_wait_barrier<(BT_NBUF - 1) * LOADS>();
+        dict_scales(t_begin, sa);
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            bres[kt][j] = bt_rd<DT>(smem + BT_RQ_QOFF + kt * BT_RQ_QSTAGE, BT_WQ * wq + 32 * j + r32, lane);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a[i] = bt_rd<DT>(smem, 32 * i + r32, lane);
+      }
+      auto stage = [&](int g, auto kt, auto first) {
+        bt_wait_barrier<2 * LOADS>();
+        const unsigned char* nbuf = smem + ((g + 1) % BT_NBUF) * STAGE;
+        constexpr int NS = DT == EBSD_DTYPE_FP8_E4M3 ? 1 : 2;
+        bt_mfma_hooked<DT, decltype(first)::value>(a, bres[decltype(kt)::value], sa, sb, acc, [&](auto kk) {
+          constexpr int k = decltype(kk)::value;
+          if constexpr (k < LOADS) issue_part(g + BT_NBUF, k);
+          if constexpr (k / 12 == NS - 1 && (k % 12) % 2 == 1)
+            a[(k % 12) / 2] = bt_rd<DT>(nbuf, 32 * ((k % 12) / 2) + r32, lane);
+        });
+      };
+      for (int tl = 0; tl < n_t; ++tl) {
+        stage(4 * tl, std::integral_constant<int, 0>{}, std::true_type{});
+        stage(4 * tl + 1, std::integral_constant<int, 1>{}, std::false_type{});
+        stage(4 * tl + 2, std::integral_constant<int, 2>{}, std::false_type{});
+        stage(4 * tl + 3, std::integral_constant<int, 3>{}, std::false_type{});
+        fold(tl);
+      }
     }
-    // the clamped loads past the last stage land before the LDS is reused
+    // The clamped loads past the last stage land, and every wave is done with
+    // the stage ring, the scales and its scratch, before the next unit's
+    // loads reuse the LDS (this wait includes lgkmcnt(0) and a barrier).
     bt_wait_barrier<0>();
-
-    // Merge the 2 lists of each query: [256 queries][2 lists][K] in LDS (no
-    // direct load is in flight: the last stage waited for all of them).
-    __syncthreads();
-    float* cv = reinterpret_cast<float*>(smem);
-    int* ci = reinterpret_cast<int*>(smem) + BT_Q * 2 * K;
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int ql = 128 * wq + 32 * (s + 2 * h) + r32;
+    // the lane's list is its query's partial list
+    const int q = q0 + BT_WQ * wq + 32 * h + r32;
+    if (q < M) {
 #pragma unroll
       for (int e = 0; e < K; ++e) {
-        cv[(ql * 2 + wd) * K + e] = tv[s][e];
-        ci[(ql * 2 + wd) * K + e] = ti[s][e];
+        const size_t o = (size_t)(part * K + e) * M + q;
+        pval[o] = tv[e];
+        pidx[o] = ti[e];
       }
     }
-    __syncthreads();
-    {
-      const int q = q0 + tid;
-      const float* v = cv + tid * 2 * K;
-      const int* ix = ci + tid * 2 * K;
-      int h0 = 0, h1 = 0;
-      for (int e = 0; e < K; ++e) {
-        // a list can only run dry on the K-th pick, so both heads stay < K
-        // whenever they are read
-        const float v0 = v[h0], v1 = v[K + h1];
-        const int i0 = ix[h0], i1 = ix[K + h1];
-        const bool take1 = topk_better(v1, i1, v0, i0);
-        if (q < M) {
-          const size_t o = (size_t)(part * K + e) * M + q;
-          pval[o] = take1 ? v1 : v0;
-          pidx[o] = take1 ? i1 : i0;
-        }
-        h0 += take1 ? 0 : 1;
-        h1 += take1 ? 1 : 0;
-      }
-    }
-    // the lists are read before the next unit's loads overwrite the LDS
-    __syncthreads();
   }
 }
 
@@ -963,8 +997,12 @@ static void launch_sim_kernel(const SimPlan& p, const unsigned char* Qb, const f
                               int* pidx, int index_base, hipStream_t st) {
   if constexpr (DT != EBSD_DTYPE_F32) {
     if (p.bt) {
-      hipLaunchKernelGGL((sim_topk_bt_kernel<DT, K>), dim3(p.grid), dim3(BT_THREADS), 0, st, Qb, q_scale, m, Db,
-                         d_scale, n, ld_bytes, p.sc, pval, pidx, index_base);
+      if (ld_bytes == BT_RQ_ROW_BYTES)
+        hipLaunchKernelGGL((sim_topk_bt_kernel<DT, K, true>), dim3(p.grid), dim3(BT_THREADS), 0, st, Qb, q_scale,
+                           m, Db, d_scale, n, ld_bytes, p.sc, pval, pidx, index_base);
+      else
+        hipLaunchKernelGGL((sim_topk_bt_kernel<DT, K, false>), dim3(p.grid), dim3(BT_THREADS), 0, st, Qb, q_scale,
+                           m, Db, d_scale, n, ld_bytes, p.sc, pval, pidx, index_base);
       return;
     }
   }
