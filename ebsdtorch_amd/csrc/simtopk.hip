@@ -451,15 +451,44 @@ __device__ __forceinline__ unsigned bt_spread_rows(unsigned p) {
   return (y | (y << 4)) & 0x0f0f0f0fu;
 }
 
+// Offer queues of the resident-query mode: the query block of the LDS is free
+// once the query operands sit in registers, and holds per lane a FIFO of
+// BT_QCAP offers (score, index bits), slot s of lane l at qbuf[64 s + l] of
+// its wave's part. Marked scores are queued at a few instructions each and
+// inserted into the list in batches (bt_drain): every BT_DRAIN_TILES tiles,
+// at the end of a work unit, and whenever a queue could overflow. Batching
+// lets one insertion pass serve the marks of many lanes, and gives every wave
+// the same insertion work at the same tiles, so the waves wait less for each
+// other at the stage barriers.
+constexpr int BT_QCAP = 32;
+constexpr int BT_DRAIN_TILES = 16;
+static_assert(4 * 64 * BT_QCAP * 8 <= 4 * BT_RQ_QSTAGE, "the offer queues fit in the query block");
+
+// Offer the lane's queued entries to its list in queue order (ascending
+// index, as offers arrive), the next one read ahead; empties the queue.
+template <int K>
+__device__ __forceinline__ void bt_drain(float (&tv)[K], int (&ti)[K], const float2* qbuf, int& qlen, int lane) {
+  float2 e = qbuf[lane];
+  for (int p = 0; p < qlen; ++p) {
+    const float2 en = qbuf[64 * min(p + 1, BT_QCAP - 1) + lane];
+    topk_offer_ascending<K>(tv, ti, e.x, __float_as_int(e.y));
+    e = en;
+  }
+  qlen = 0;
+}
+
 // Fold a finished 192 x 64 wave tile into the lane's list. Lane (h, c)
 // (h = lane / 32, c = lane % 32) holds, of every 32 x 32 tile, column c and
 // rows 8 q + 4 h + rr (accumulator register 4 q + rr); its list belongs to
 // column c of column block h. dbase: index of the tile's first dictionary
 // row; n_valid: how many of its 192 rows exist; scratch: the wave's 64
-// scratch rows. Offers reach the list in ascending dictionary index.
-template <int K>
+// scratch rows. Offers reach the list in ascending dictionary index. QUEUE:
+// marked scores go to the lane's offer queue (qbuf, qlen) instead of straight
+// into the list, and the bound of a column is its list's K-th best without the
+// queued offers (a lower bound, so nothing that belongs is screened out).
+template <int K, bool QUEUE>
 __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)[K], int (&ti)[K], int dbase,
-                                        int n_valid, float* scratch, int lane) {
+                                        int n_valid, float* scratch, int lane, float2* qbuf, int& qlen) {
   const int h = lane >> 5, c = lane & 31;
   // the K-th best scores of the two columns whose scores the lane holds:
   // column c of block 0 (the list of lane c) and of block 1 (lane 32 + c)
@@ -470,19 +499,22 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)
   // Screen every 32-row block first, with no branch in between: bit 2 i
   // (resp. 2 i + 1) is set when some lane may take a score of column block 0
   // (resp. 1) in row block i.
-  unsigned hits = 0;
+  // (kept as the ballots themselves and tested where they are used, so the
+  // test stays a scalar compare)
+  unsigned long long hit_a[6], hit_b[6];
+  unsigned long long any = 0;
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     const float ma = bt_max16(acc[i][0]);
     const float mb = bt_max16(acc[i][1]);
-    if (__builtin_amdgcn_ballot_w64(!(ma < ta)) != 0) hits |= 1u << (2 * i);
-    if (__builtin_amdgcn_ballot_w64(!(mb < tb)) != 0) hits |= 2u << (2 * i);
+    hit_a[i] = __builtin_amdgcn_ballot_w64(!(ma < ta));
+    hit_b[i] = __builtin_amdgcn_ballot_w64(!(mb < tb));
+    any |= hit_a[i] | hit_b[i];
   }
-  if (hits == 0) return;
+  if (any == 0) return;
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    const unsigned hb = (hits >> (2 * i)) & 3u;
-    if (hb == 0) continue;
+    if ((hit_a[i] | hit_b[i]) == 0) continue;
     // Some lane may take a score: write the scores of the column block (or
     // blocks) concerned into the scratch rows of the lanes that own their
     // columns (row 8 q + 4 h + rr at float 8 q + 4 h + rr), mark the ones
@@ -493,7 +525,7 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)
     // values), shifted in from j = 15 down to j = 0; a block that is not
     // staged fails everywhere
     unsigned fail_a = 0xffffu, fail_b = 0xffffu;
-    if (hb & 1u) {
+    if (hit_a[i] != 0) {
 #pragma unroll
       for (int q = 3; q >= 0; --q) {
         floatx4 va;
@@ -504,7 +536,7 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)
         for (int rr = 3; rr >= 0; --rr) fail_a = __builtin_amdgcn_alignbit(fail_a, __float_as_uint(va[rr] - ta), 31);
       }
     }
-    if (hb & 2u) {
+    if (hit_b[i] != 0) {
 #pragma unroll
       for (int q = 3; q >= 0; --q) {
         floatx4 vb;
@@ -524,6 +556,23 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)
     // The partner's stores precede these loads in the wave's LDS order; the
     // clobbers keep the compiler from moving memory accesses across.
     asm volatile("" ::: "memory");
+    if constexpr (QUEUE) {
+      // make room first if some lane's marks could overflow its queue (after
+      // a drain, 32 marks fit)
+      if (__builtin_amdgcn_ballot_w64(qlen + __builtin_popcount(m) > BT_QCAP) != 0)
+        bt_drain<K>(tv, ti, qbuf, qlen, lane);
+      while (m != 0) {
+        const int rho = __builtin_ctz(m);
+        m &= m - 1;
+        const int row = 32 * i + rho;
+        if (row < n_valid) {
+          qbuf[64 * qlen + lane] = make_float2(scratch[BT_SCR_ROW * lane + rho], __int_as_float(dbase + row));
+          ++qlen;
+        }
+      }
+      asm volatile("" ::: "memory");
+      continue;
+    }
     // the score of the next marked row is read ahead of the current row's
     // insertion (for an empty rest, row 31's: a harmless in-bounds read)
     int rho = __builtin_ctz(m | 0x80000000u);
@@ -645,25 +694,28 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
           bt_dma16(qsrc[j] + kt * BT_KB, lds0 + BT_RQ_QOFF + kt * BT_RQ_QSTAGE + (wave_s + 4 * j) * 1024);
     };
     // E8M0 exponents of the lane's dictionary rows of tile t (byte i % 4 of
-    // s[i / 4]: block i). The results pass through a move where they are
-    // read, so their LDS wait is settled here and not at the MFMAs of the
-    // next stage.
+    // s[i / 4]: block i)
     auto dict_scales = [&](int t, int (&s)[2]) {
       s[0] = s[1] = 0;
       if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
         const float* sl = sscale + (t & 1) * 256 + r32;
 #pragma unroll
         for (int i = 0; i < 6; ++i) s[i >> 2] |= e8m0_of(sl[32 * i]) << (8 * (i & 3));
-        asm volatile("v_mov_b32 %0, %1" : "=v"(s[0]) : "v"(s[0]));
-        asm volatile("v_mov_b32 %0, %1" : "=v"(s[1]) : "v"(s[1]));
       }
     };
 
     int sa[2] = {0, 0};
     const int n_t = G / max(nk, 1);
+    // RQ: the wave's offer queues in the query block (see BT_QCAP)
+    float2* qbuf = reinterpret_cast<float2*>(smem + BT_RQ_QOFF) + wave * 64 * BT_QCAP;
+    int qlen = 0;
     auto fold = [&](int tl) {
       const int d0 = (t_begin + tl) * BT_D;
-      bt_fold<K>(acc, tv, ti, index_base + d0, N - d0, scratch, lane);
+      bt_fold<K, RQ>(acc, tv, ti, index_base + d0, N - d0, scratch, lane, qbuf, qlen);
+      if constexpr (RQ) {
+        if ((tl % BT_DRAIN_TILES == BT_DRAIN_TILES - 1 || tl + 1 == n_t) && __builtin_amdgcn_ballot_w64(qlen > 0) != 0)
+          bt_drain<K>(tv, ti, qbuf, qlen, lane);
+      }
       // the next tile's scales came with this tile's stages
       if (tl + 1 < n_t) dict_scales(t_begin + tl + 1, sa);
       // no LDS operation of the fold stays in flight into the next tile's
