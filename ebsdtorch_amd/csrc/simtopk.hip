@@ -814,6 +814,323 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-workgroups-per-CU variant for rows of 256 bytes (sim_topk_b2_kernel):
+// a wave's vector instructions do not overlap its own matrix instructions,
+// but they do overlap those of another wave on the same SIMD. With one
+// 4-wave workgroup per CU the fold of every tile (screen, staging, list
+// insertion) adds to the MFMA time; here each CU runs two independent 4-wave
+// workgroups (one wave of each per SIMD), each with its own stage ring and
+// barriers, so that one workgroup's fold runs while the other's MFMAs do.
+//
+//   block   : 256 threads = 4 waves, two workgroups per CU (512 persistent
+//             workgroups = 8 XCDs x 64 slots); block tile 192 dictionary rows
+//             x 128 queries; wave w owns queries [32 w, 32 w + 32) against all
+//             192 rows: 6 MFMA 32x32 tiles (96 accumulator registers), so that
+//             the whole wave fits in the 256 registers of two waves per SIMD.
+//             Every score of a query is made in one wave: one list per query,
+//             held by lane c < 32 of its column (the other lane half holds
+//             the column's other 16 rows of each tile and no list)
+//   operands: the unit's 128 query rows are loaded once (through the stage
+//             ring's memory, before the ring starts) into registers: each
+//             wave's query operands of all four stages (32 VGPRs). The ring
+//             carries the 192 dictionary rows (12 KiB per stage, three 1-KiB
+//             direct loads per wave plus the scales); the dictionary operands
+//             of stage g + 1 are read during stage g's MFMAs, block i as soon
+//             as its MFMA is issued. Stage g's barrier means: stage g + 1 has
+//             landed and every wave has read stage g, whose buffer then takes
+//             stage g + 4
+//   top-k   : the screen, staging and offer queues of sim_topk_bt_kernel's
+//             resident mode, for one column block per wave (queues only for
+//             the 32 list-holding lanes)
+//   LDS     : 48 KiB ring (first the query block), 8 KiB queues, 2 KiB
+//             scales, 18 KiB fold scratch: 76 KiB per workgroup
+constexpr int B2_Q = 128;                                  // queries per tile
+constexpr int B2_WQ = 32;                                  // queries per wave
+constexpr int B2_QCAP = 8;                                 // queue slots per list
+constexpr int B2_QUEUE_OFF = BT_NBUF * BT_RQ_STAGE;        // 48 KiB
+constexpr int B2_SCALE_OFF = B2_QUEUE_OFF + 4 * 32 * B2_QCAP * 8;
+constexpr int B2_SCR_OFF = B2_SCALE_OFF + 2 * 256 * 4;
+constexpr int B2_LDS = B2_SCR_OFF + 4 * 32 * BT_SCR_ROW * 4;
+constexpr int B2_SLOTS_PER_XCD = 64;
+constexpr int B2_ASLOTS = 2;                               // dictionary operand registers (x 8)
+static_assert(B2_LDS <= 80 * 1024, "two workgroups per CU");
+static_assert(B2_Q * BT_RQ_ROW_BYTES <= B2_QUEUE_OFF, "the query block is loaded through the ring");
+
+// Largest of the 16 accumulator registers of one 32x32 tile in this lane, for
+// accumulators the compiler keeps in VGPRs. Written with fmaxf, not asm: the
+// values come straight from MFMAs, and only compiler-visible reads get the
+// wait states an MFMA result needs before a vector instruction may read it
+// (the MFMA results are canonical, so fmaxf adds no quieting).
+__device__ __forceinline__ float b2_max16(const floatx16& a) {
+  float m[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) m[r] = __builtin_elementwise_maximum(a[2 * r], a[2 * r + 1]);
+#pragma unroll
+  for (int w = 4; w > 0; w >>= 1)
+#pragma unroll
+    for (int r = 0; r < w; ++r) m[r] = __builtin_elementwise_maximum(m[r], m[r + w]);
+  return m[0];
+}
+
+// Offer the lane's queued entries (slot s at qbuf[32 s + c]) to its list in
+// queue order, the next one read ahead; empties the queue.
+template <int K>
+__device__ __forceinline__ void b2_drain(float (&tv)[K], int (&ti)[K], const float2* qbuf, int& qlen, int c) {
+  float2 e = qbuf[c];
+  for (int p = 0; p < qlen; ++p) {
+    const float2 en = qbuf[32 * min(p + 1, B2_QCAP - 1) + c];
+    topk_offer_ascending<K>(tv, ti, e.x, __float_as_int(e.y));
+    e = en;
+  }
+  qlen = 0;
+}
+
+// Fold a finished 192 x 32 wave tile. Lane (h, c) holds column c and rows
+// 8 q + 4 h + rr of every 32 x 32 tile; lane c (h = 0) owns the column's
+// list. Marked scores are staged in the owner's scratch row and queued.
+template <int K>
+__device__ __forceinline__ void b2_fold(const floatx16 (&acc)[6], float (&tv)[K], int (&ti)[K], int dbase,
+                                        int n_valid, float* scratch, int lane, float2* qbuf, int& qlen) {
+  const int h = lane >> 5, c = lane & 31;
+  // the column's K-th best (lane c's) in both lane halves
+  const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(tv[K - 1]), __float_as_uint(tv[K - 1]), false,
+                                                  false);
+  const float th = __uint_as_float(t[0]);
+  // screen: one ballot per 32-row block, kept as a scalar mask
+  unsigned long long hit[6];
+  unsigned long long any = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    hit[i] = __builtin_amdgcn_ballot_w64(!(b2_max16(acc[i]) < th));
+    any |= hit[i];
+  }
+  if (any == 0) return;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if (hit[i] == 0) continue;
+    // stage the block's scores in the owner's row (row 8 q + 4 h + rr at
+    // float 8 q + 4 h + rr) and mark the ones that pass the bound (fail bit
+    // 4 q + rr: sign of score - bound, shifted in from bit 15 down)
+    unsigned fail = 0xffffu;
+#pragma unroll
+    for (int q = 3; q >= 0; --q) {
+      floatx4 v;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) v[rr] = acc[i][4 * q + rr];
+      *reinterpret_cast<floatx4*>(scratch + BT_SCR_ROW * c + 8 * q + 4 * h) = v;
+#pragma unroll
+      for (int rr = 3; rr >= 0; --rr) fail = __builtin_amdgcn_alignbit(fail, __float_as_uint(v[rr] - th), 31);
+    }
+    const unsigned bits = bt_spread_rows(~fail & 0xffffu) << (4 * h);
+    // the owner's marks: its own rows and those of lane 32 + c
+    const auto o = __builtin_amdgcn_permlane32_swap(bits, bits, false, false);
+    unsigned m = h == 0 ? (o[0] | o[1]) : 0u;
+    // the partner's stores precede these loads in the wave's LDS order
+    asm volatile("" ::: "memory");
+    if (__builtin_amdgcn_ballot_w64(qlen + __builtin_popcount(m) > B2_QCAP) != 0) {
+      // a queue would overflow: insert the queued offers, then this block's
+      // marks straight away (up to 32 of them, more than a queue holds)
+      b2_drain<K>(tv, ti, qbuf, qlen, c);
+      while (m != 0) {
+        const int rho = __builtin_ctz(m);
+        m &= m - 1;
+        const int row = 32 * i + rho;
+        topk_offer_ascending<K>(tv, ti, row < n_valid ? scratch[BT_SCR_ROW * c + rho] : -INFINITY, dbase + row);
+      }
+    } else {
+      while (m != 0) {
+        const int rho = __builtin_ctz(m);
+        m &= m - 1;
+        const int row = 32 * i + rho;
+        if (row < n_valid) {
+          qbuf[32 * qlen + c] = make_float2(scratch[BT_SCR_ROW * c + rho], __int_as_float(dbase + row));
+          ++qlen;
+        }
+      }
+    }
+    asm volatile("" ::: "memory");
+  }
+}
+
+// The six MFMAs (dictionary blocks I .. 5, block i's operand in ar[i % 3]) of
+// K step S of one stage for the wave's query block b; hook(I) after the MFMA
+// of block I. (FP8 only takes the two-workgroup kernel: one step.)
+template <int DT, int S, bool Z, int I = 0, typename Hook>
+__device__ __forceinline__ void b2_mfma_step(const intx8 (&ar)[B2_ASLOTS], const intx8& b, const int (&sa)[2], int sb,
+                                             floatx16 (&acc)[6], Hook&& hook) {
+  if constexpr (I < 6) {
+    const floatx16 c = Z ? floatx16{} : acc[I];
+    const intx8& a = ar[I % B2_ASLOTS];
+    if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+      acc[I] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, I & 3, sa[I >> 2], 0, sb);
+    } else {
+      const intx4 a4 = {a[4 * S], a[4 * S + 1], a[4 * S + 2], a[4 * S + 3]};
+      const intx4 b4 = {b[4 * S], b[4 * S + 1], b[4 * S + 2], b[4 * S + 3]};
+      if constexpr (DT == EBSD_DTYPE_BF16)
+        acc[I] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a4), __builtin_bit_cast(bf16x8, b4),
+                                                         c, 0, 0, 0);
+      else
+        acc[I] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(halfx8, a4), __builtin_bit_cast(halfx8, b4),
+                                                        c, 0, 0, 0);
+    }
+    hook(std::integral_constant<int, I>{});
+    b2_mfma_step<DT, S, Z, I + 1>(ar, b, sa, sb, acc, hook);
+  }
+}
+
+// Requires N >= BT_D and ld_bytes == BT_RQ_ROW_BYTES (4 stages); see plan_sim.
+template <int DT, int K>
+__global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
+    const unsigned char* __restrict__ Q, const float* __restrict__ q_scale, int M,
+    const unsigned char* __restrict__ Dm, const float* __restrict__ d_scale, int N, int ld_bytes,
+    SimSched sc, float* __restrict__ pval, int* __restrict__ pidx, int index_base) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[B2_LDS];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wq = wave;
+  const int r32 = lane & 31, h = lane >> 5;
+  constexpr int NK = BT_RQ_ROW_BYTES / BT_KB;  // 4 stages per tile
+  constexpr int LOADS = BT_RQ_LOADS;
+  constexpr int STAGE = BT_RQ_STAGE;
+  constexpr int NS = DT == EBSD_DTYPE_FP8_E4M3 ? 1 : 2;
+  // 64 slots per XCD (two workgroups per CU); the mapping is slot_work's
+  const SlotWork wk = slot_work(sc, blockIdx.x, (N + BT_D - 1) / BT_D);
+  if (wk.idle) return;
+  const int part = wk.part;
+  const int t_begin = wk.t_begin;
+  const int t_end = wk.t_end;
+  const int n_t = max(t_end - t_begin, 0);
+  // direct-load lane mapping as in sim_topk_bt_kernel: load L writes rows
+  // 16 L .. 16 L + 15, lane -> row 16 L + lane / 4 at physical chunk lane % 4
+  const int lrow = lane >> 2;
+  const int lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  const float* sscale = reinterpret_cast<const float*>(smem + B2_SCALE_OFF);
+  float* scratch = reinterpret_cast<float*>(smem + B2_SCR_OFF) + wave * 32 * BT_SCR_ROW;
+  float2* qbuf = reinterpret_cast<float2*>(smem + B2_QUEUE_OFF) + wave * 32 * B2_QCAP;
+  const unsigned lds0 = (unsigned)(uintptr_t)(lds_ptr_t)smem;
+  const unsigned wave_s = __builtin_amdgcn_readfirstlane(wave);
+
+  for (int grp = wk.g_first; grp < sc.n_qgroups; grp += wk.g_step) {
+    const int qt = grp * sc.A + wk.a;
+    if (qt >= sc.n_qtiles) continue;  // uniform over the workgroup
+    const int q0 = qt * B2_Q;
+    // FP8: E8M0 exponent of the lane's query row
+    int sb = 0;
+    if constexpr (DT == EBSD_DTYPE_FP8_E4M3) sb = e8m0_of(q_scale[min(q0 + B2_WQ * wq + r32, M - 1)]);
+    float tv[K];
+    int ti[K];
+    topk_init<K>(tv, ti);
+    floatx16 acc[6];
+    int qlen = 0;
+
+    auto scale_src = [&](int t) -> const void* {
+      if constexpr (DT == EBSD_DTYPE_FP8_E4M3)
+        return d_scale + min(t * BT_D + 64 * wave + lane, N - 1);
+      else
+        return Dm + (size_t)min(t * BT_D, N - 1) * ld_bytes + 4 * lane;
+    };
+    auto scale_m0 = [&](int t) { return lds0 + B2_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
+    // loads of stage g: k < 3 dictionary rows 16 (wave + 4 k) .., k = 3 the
+    // scales of the tile after g's; addresses clamped, also for g past the end
+    auto issue_part = [&](int g, int k) {
+      const int tl = g / NK;
+      const int kt = g - tl * NK;
+      const int t = t_begin + tl;
+      if (k < 3) {
+        const int L = wave + 4 * k;
+        const unsigned char* src =
+            Dm + (size_t)min(t * BT_D + 16 * L + lrow, N - 1) * ld_bytes + lchunk * 16 + kt * BT_KB;
+        bt_dma16(src, lds0 + (g % BT_NBUF) * STAGE + (wave_s + 4 * k) * 1024);
+      } else {
+        bt_dma4(scale_src(t + 1), scale_m0(t + 1));
+      }
+    };
+    auto dict_scales = [&](int t, int (&s)[2]) {
+      s[0] = s[1] = 0;
+      if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+        const float* sl = sscale + (t & 1) * 256 + r32;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i >> 2] |= e8m0_of(sl[32 * i]) << (8 * (i & 3));
+      }
+    };
+
+    int sa[2] = {0, 0};
+    intx8 bres[NK];
+    // dictionary operands rotate through three registers: block i of a stage
+    // in ar[i % 3]; after the MFMA of block i, block i + 3 of the same stage
+    // (i < 3) or block i - 3 of the next (i >= 3) is read into its place.
+    // Stage g's buffer is read during stages g - 1 and g, so stage g's loads
+    // go into the buffer of stage g - 1 (issued during stage g - 3: a lead of
+    // two stages) and every wait leaves one batch in flight.
+    intx8 ar[B2_ASLOTS];
+    if (n_t > 0) {
+      // the unit's 128 query rows, stage kt's 64 bytes of them at kt * 8 KiB
+      // (the ring's memory, before the ring starts): 8 loads per wave
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const unsigned char* qs = Q + (size_t)min(q0 + 16 * (wave + 4 * j) + lrow, M - 1) * ld_bytes + lchunk * 16;
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) bt_dma16(qs + kt * BT_KB, lds0 + kt * B2_Q * BT_KB + (wave_s + 4 * j) * 1024);
+      }
+      bt_wait_barrier<0>();
+#pragma unroll
+      for (int kt = 0; kt < NK; ++kt) bres[kt] = bt_rd<DT>(smem + kt * B2_Q * BT_KB, B2_WQ * wq + r32, lane);
+      // every wave has its query operands before the ring overwrites them
+      bt_wait_barrier<0>();
+      // the first tile's scales and stages 0 .. 2
+      bt_dma4(scale_src(t_begin), scale_m0(t_begin));
+#pragma unroll
+      for (int p = 0; p < BT_NBUF - 1; ++p)
+#pragma unroll
+        for (int k = 0; k < LOADS; ++k) issue_part(p, k);
+      bt_wait_barrier<2 * LOADS>();
+      dict_scales(t_begin, sa);
+#pragma unroll
+      for (int i = 0; i < B2_ASLOTS; ++i) ar[i] = bt_rd<DT>(smem, 32 * i + r32, lane);
+    }
+    auto stage = [&](int tl, auto kt, auto first) {
+      const int g = NK * tl + decltype(kt)::value;
+      bt_wait_barrier<LOADS>();
+      const unsigned char* cbuf = smem + (g % BT_NBUF) * STAGE;
+      const unsigned char* nbuf = smem + ((g + 1) % BT_NBUF) * STAGE;
+      b2_mfma_step<DT, 0, decltype(first)::value>(ar, bres[decltype(kt)::value], sa, sb, acc, [&](auto i) {
+        constexpr int I = decltype(i)::value;
+        if constexpr (I < LOADS) issue_part(g + BT_NBUF - 1, I);
+        if constexpr (I + B2_ASLOTS < 6)
+          ar[I % B2_ASLOTS] = bt_rd<DT>(cbuf, 32 * (I + B2_ASLOTS) + r32, lane);
+        else
+          ar[I % B2_ASLOTS] = bt_rd<DT>(nbuf, 32 * (I + B2_ASLOTS - 6) + r32, lane);
+      });
+    };
+    for (int tl = 0; tl < n_t; ++tl) {
+      stage(tl, std::integral_constant<int, 0>{}, std::true_type{});
+      stage(tl, std::integral_constant<int, 1>{}, std::false_type{});
+      stage(tl, std::integral_constant<int, 2>{}, std::false_type{});
+      stage(tl, std::integral_constant<int, 3>{}, std::false_type{});
+      const int d0 = (t_begin + tl) * BT_D;
+      b2_fold<K>(acc, tv, ti, index_base + d0, N - d0, scratch, lane, qbuf, qlen);
+      if ((tl % BT_DRAIN_TILES == BT_DRAIN_TILES - 1 || tl + 1 == n_t) && __builtin_amdgcn_ballot_w64(qlen > 0) != 0)
+        b2_drain<K>(tv, ti, qbuf, qlen, r32);
+      if (tl + 1 < n_t) dict_scales(t_begin + tl + 1, sa);
+      __builtin_amdgcn_s_waitcnt(0xC07F);
+    }
+    // the clamped loads past the last stage land and every wave is done with
+    // the LDS before the next unit's query rows overwrite the ring
+    bt_wait_barrier<0>();
+    const int q = q0 + B2_WQ * wq + r32;
+    if (h == 0 && q < M) {
+#pragma unroll
+      for (int e = 0; e < K; ++e) {
+        const size_t o = (size_t)(part * K + e) * M + q;
+        pval[o] = tv[e];
+        pidx[o] = ti[e];
+      }
+    }
+  }
+}
+
 // Merge n_splits sorted partial lists (layout [split][K][M]) and, if `merge`,
 // the caller's running state (M, topk), into the sorted state (M, topk).
 template <int K>
@@ -908,7 +1225,8 @@ __global__ __launch_bounds__(256) void rescore_kernel(
 
 struct SimPlan {
   int k_list;   // register list length (1 or 16)
-  bool bt;      // 192 x 256 low-precision kernel (sim_topk_bt_kernel) instead of the 128 x 128 one
+  bool bt;      // a low-precision large-tile kernel instead of the 128 x 128 one
+  bool b2;      // ... the two-workgroups-per-CU one (sim_topk_b2_kernel)
   SimSched sc;
   int n_parts;  // partial lists per query (= B * R)
   int grid;
@@ -947,7 +1265,12 @@ SimPlan plan_sim(int m, int n, int topk, int dtype, int ld_bytes) {
   p.k_list = topk <= 1 ? 1 : EBSD_MAX_TOPK;
   SimSched& s = p.sc;
   p.bt = dtype != EBSD_DTYPE_F32 && n >= BT_D && ld_bytes / BT_KB >= BT_MIN_STAGES;
-  if (p.bt) {
+  p.b2 = p.bt && ld_bytes == BT_RQ_ROW_BYTES && dtype == EBSD_DTYPE_FP8_E4M3;
+  if (p.b2) {
+    s.n_qtiles = (m + B2_Q - 1) / B2_Q;
+    plan_groups(s, B2_SLOTS_PER_XCD, B2_SLOTS_PER_XCD, (n + BT_D - 1) / BT_D, p.n_parts);
+    p.grid = kXcds * B2_SLOTS_PER_XCD;
+  } else if (p.bt) {
     s.n_qtiles = (m + BT_Q - 1) / BT_Q;
     plan_groups(s, BT_SLOTS_PER_XCD, ld_bytes <= kBtShortRowBytes ? kBtGroupTilesShort : kBtGroupTiles,
                 (n + BT_D - 1) / BT_D, p.n_parts);
@@ -1013,10 +1336,10 @@ extern "C" int ebsd_sim_schedule(int m, int n, int topk, int dtype, int ld, int*
   if (eb == 0) return EBSD_ERR_DTYPE;
   if (ld % ebsd_pad_multiple(dtype) != 0) return EBSD_ERR_SHAPE;
   const SimPlan p = plan_sim(m, n, topk, dtype, ld * eb);
-  const int q_rows = p.bt ? BT_Q : TILE;
+  const int q_rows = p.b2 ? B2_Q : p.bt ? BT_Q : TILE;
   const int d_rows = p.bt ? BT_D : TILE;
   const int n_dtiles = (n + d_rows - 1) / d_rows;
-  const int vals[EBSD_SCHEDULE_INFO] = {p.bt ? 1 : 0, p.grid, p.n_parts, p.k_list, q_rows,
+  const int vals[EBSD_SCHEDULE_INFO] = {p.b2 ? 2 : p.bt ? 1 : 0, p.grid, p.n_parts, p.k_list, q_rows,
                                         d_rows, p.sc.n_qtiles, n_dtiles, p.sc.A, p.sc.B, p.sc.R};
   for (int i = 0; i < EBSD_SCHEDULE_INFO; ++i) info[i] = vals[i];
   const long long nv = (long long)p.sc.n_qtiles * n_dtiles;
@@ -1048,6 +1371,13 @@ static void launch_sim_kernel(const SimPlan& p, const unsigned char* Qb, const f
                               const unsigned char* Db, const float* d_scale, int n, int ld_bytes, float* pval,
                               int* pidx, int index_base, hipStream_t st) {
   if constexpr (DT != EBSD_DTYPE_F32) {
+    if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
+      if (p.b2) {
+        hipLaunchKernelGGL((sim_topk_b2_kernel<DT, K>), dim3(p.grid), dim3(BT_THREADS), 0, st, Qb, q_scale, m, Db,
+                           d_scale, n, ld_bytes, p.sc, pval, pidx, index_base);
+        return;
+      }
+    }
     if (p.bt) {
       if (ld_bytes == BT_RQ_ROW_BYTES)
         hipLaunchKernelGGL((sim_topk_bt_kernel<DT, K, true>), dim3(p.grid), dim3(BT_THREADS), 0, st, Qb, q_scale,

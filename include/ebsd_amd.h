@@ -208,7 +208,8 @@ int ebsd_sim_topk_workspace(int m, int n, int topk, int dtype, size_t* bytes);
 /*
  * Work schedule of ebsd_sim_topk for these sizes, walked on the host with the
  * kernels' own block -> work mapping (no device work). info receives
- * EBSD_SCHEDULE_INFO ints: kernel (0 = 128 x 128, 1 = 192 x 256), grid
+ * EBSD_SCHEDULE_INFO ints: kernel (0 = 128 x 128, 1 = 192 x 256, 2 = 192 x 128 with
+ * two workgroups per CU), grid
  * (workgroups), partial lists per query, list length, query rows per tile,
  * dictionary rows per tile, query tiles, dictionary tiles, and the grouping
  * A, B, R. If given, tile_visits (query tiles x dictionary tiles, row-major)

@@ -1,8 +1,8 @@
 """The persistent similarity kernels' work plan (ebsd_sim_schedule), walked
 on the host with the kernels' own block -> work mapping: every (query tile,
 dictionary tile) pair is scored by exactly one workgroup and every partial
-top-k list that the merge reads is written exactly once, for both kernels,
-both groupings of the 192 x 256 one and the shapes of the benchmark
+top-k list that the merge reads is written exactly once, for all three
+kernels, both groupings of the 192 x 256 one and the shapes of the benchmark
 configurations. No GPU is needed."""
 
 import ctypes
@@ -28,7 +28,9 @@ CASES = [
     (300, 191, 3600, FP8, 16),            # below one 192-row tile: 128 x 128 kernel
     (300, 192, 3600, FP8, 16),            # one 192-row tile
     (300, 5000, 128, FP8, 16),            # 128-byte rows: too few stages, 128 x 128 kernel
-    (65_536, 995_277, 256, FP8, 16),      # 256 principal coordinates: short-row grouping
+    (65_536, 995_277, 256, FP8, 16),      # 256 principal coordinates: two workgroups per CU
+    (300, 5000, 256, FP8, 16),            # ... with fewer query groups than XCDs
+    (65_536, 995_277, 128, BF16, 16),     # 256-byte BF16 rows: resident queries, one workgroup per CU
     (65_536, 995_277, 448, FP8, 16),
     (65_536, 995_277, 3600, FP8, 16),     # full-pattern FP8 at 60x60
     (65_536, 995_277, 3600, BF16, 16),
@@ -44,10 +46,14 @@ def test_every_tile_pair_and_partial_list_once(m, n, n_pix, dtype, topk):
     ld = padded_width(n_pix, dtype)
     s = sim_schedule(m, n, topk, dtype, ld)
     eb = {F32: 4, BF16: 2, F16: 2, FP8: 1}[dtype]
-    # which kernel: the 192 x 256 one for low precision with >= 192 rows and
-    # >= 4 stages of 64 bytes
-    assert s["bt"] == int(dtype != F32 and n >= 192 and ld * eb // 64 >= 4)
-    assert (s["q_rows"], s["d_rows"]) == ((256, 192) if s["bt"] else (128, 128))
+    # which kernel: a large-tile one for low precision with >= 192 rows and
+    # >= 4 stages of 64 bytes; FP8 rows of exactly 256 bytes take the
+    # 192 x 128 kernel with two workgroups per CU (2), the others the
+    # 192 x 256 one (1)
+    large = dtype != F32 and n >= 192 and ld * eb // 64 >= 4
+    two = large and dtype == FP8 and ld * eb == 256
+    assert s["bt"] == (2 if two else 1 if large else 0)
+    assert (s["q_rows"], s["d_rows"]) == ((128, 192) if two else (256, 192) if large else (128, 128))
     assert s["n_qtiles"] == -(-m // s["q_rows"]) and s["n_dtiles"] == -(-n // s["d_rows"])
     # the grid is 8 XCDs x (A x B) slots
     assert s["grid"] == 8 * s["A"] * s["B"]
