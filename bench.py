@@ -170,6 +170,7 @@ def main():
         + (f"; search on {eng.search_components} principal components ({eng.explained:.4f} of the second moment)"
            if eng.search_components else ""))
 
+    head_kernel = sim_kernel_label(eng, a.batch, max(a.topk, rk))
     head = time_form(eng, exp, a.topk, a.steps, a.warmup)
     value = head["rate"]
     log(f"{a.precision}: {head['ms_per_step']:.1f} ms/step, similarity {head['sim_ms']:.1f} ms, {value:.0f} patterns/s")
@@ -290,8 +291,7 @@ def main():
         },
         "forms": forms,
         "roofline": {
-            "kernel": "sim_topk (similarity GEMM + fused top-k)" + (
-                " sim_topk_bt_kernel 192x256" if a.precision != "fp32" else " sim_topk_kernel 128x128"),
+            "kernel": "sim_topk (similarity GEMM + fused top-k) " + head_kernel,
             "bound": "mfma",
             "achieved": round(achieved, 2),
             "peak": peak,
@@ -433,7 +433,8 @@ def sub_form(mp, rays, dico, exp, ho_exp, ref, q32, fp32, precision, components,
         "search": (f"{eng.search_components} principal components ({eng.explained:.4f} of the second moment)"
                    if eng.search_components else f"full {exp.shape[1]} pixels"),
         "patterns_per_s": round(r["rate"], 1), "ms_per_step": round(r["ms_per_step"], 2), "steps": steps,
-        "similarity_kernel": {"avg_launch_ms": round(r["sim_ms"], 3), "achieved_TFLOPs": round(tf, 1),
+        "similarity_kernel": {"name": sim_kernel_label(eng, exp.shape[0], max(a.topk, 16)),
+                              "avg_launch_ms": round(r["sim_ms"], 3), "achieved_TFLOPs": round(tf, 1),
                               "peak_TFLOPs": PEAKS[precision], "mfma_frac": round(tf / PEAKS[precision], 4),
                               "contraction_dim": k},
         "top1_agreement_vs_fp32": float((r["idx"][:, 0] == fp32["idx"][:, 0]).float().mean()),
@@ -449,6 +450,17 @@ def sub_form(mp, rays, dico, exp, ho_exp, ref, q32, fp32, precision, components,
         f"{out['top1_agreement_vs_fp32']:.5f} / held out {out['heldout_top1_agreement_vs_fp32']:.5f}")
     del eng
     return out
+
+
+def sim_kernel_label(eng, m, topk):
+    """Name and tile of the similarity kernel the library launches for m
+    queries against the engine's search matrix (from ebsd_sim_schedule)."""
+    from ebsdtorch_amd import kernels
+
+    d = eng.dictionary
+    plan = kernels.sim_schedule(m, d.rows, topk, d.dtype, d.ld, counts=False)
+    name = {2: "sim_topk_b2_kernel", 1: "sim_topk_bt_kernel", 0: "sim_topk_kernel"}[plan["bt"]]
+    return f"{name} {plan['d_rows']}x{plan['q_rows']}"
 
 
 def lookup_traffic(precision, n_dict, batch, detector, topk, search_dim, D):

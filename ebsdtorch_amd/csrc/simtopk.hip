@@ -288,6 +288,16 @@ __device__ __forceinline__ void bt_dma4(const void* src, unsigned m0) {
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(src), "s"(m0)
                : "memory", "m0");
 }
+// The same loads from a wave-uniform base (SGPRs) plus a 32-bit per-lane
+// byte offset: no 64-bit address arithmetic on the vector unit.
+__device__ __forceinline__ void bt_dma16_s(const void* base, unsigned off, unsigned m0) {
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(off), "s"(base), "s"(m0)
+               : "memory", "m0");
+}
+__device__ __forceinline__ void bt_dma4_s(const void* base, unsigned off, unsigned m0) {
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dword %0, %1" ::"v"(off), "s"(base), "s"(m0)
+               : "memory", "m0");
+}
 
 __device__ __forceinline__ int bt_swz(int row, int chunk) { return row * BT_KB + ((chunk ^ ((row >> 2) & 3)) << 4); }
 
@@ -862,16 +872,22 @@ static_assert(B2_Q * BT_RQ_ROW_BYTES <= B2_QUEUE_OFF, "the query block is loaded
 // values come straight from MFMAs, and only compiler-visible reads get the
 // wait states an MFMA result needs before a vector instruction may read it
 // (the MFMA results are canonical, so fmaxf adds no quieting).
-__device__ __forceinline__ float b2_max16(const floatx16& a) {
-  float m[8];
-#pragma unroll
-  for (int r = 0; r < 8; ++r) m[r] = __builtin_elementwise_maximum(a[2 * r], a[2 * r + 1]);
-#pragma unroll
-  for (int w = 4; w > 0; w >>= 1)
-#pragma unroll
-    for (int r = 0; r < w; ++r) m[r] = __builtin_elementwise_maximum(m[r], m[r + w]);
-  return m[0];
+__device__ __forceinline__ float b2_max3(float a, float b, float c) {
+  return __builtin_elementwise_maximum(__builtin_elementwise_maximum(a, b), c);
 }
+__device__ __forceinline__ float b2_max16(const floatx16& a) {
+  // eight three-way maxima (v_maximum3_f32): five over triples, two over
+  // their results and the last register, one pair
+  const float r0 = b2_max3(a[0], a[1], a[2]), r1 = b2_max3(a[3], a[4], a[5]), r2 = b2_max3(a[6], a[7], a[8]);
+  const float r3 = b2_max3(a[9], a[10], a[11]), r4 = b2_max3(a[12], a[13], a[14]);
+  return __builtin_elementwise_maximum(b2_max3(r0, r1, r2), b2_max3(r3, r4, a[15]));
+}
+
+// First dictionary row that tile t's loads read: the tile's own, except for
+// the last, partial tile, which reads the dictionary's last BT_D rows (the
+// leading ones were scored by the tile before and are skipped in the fold);
+// loads past the end read that last tile again. Requires N >= BT_D.
+__device__ __forceinline__ int b2_row0(int t, int N) { return min(t * BT_D, N - BT_D); }
 
 // Offer the lane's queued entries (slot s at qbuf[32 s + c]) to its list in
 // queue order, the next one read ahead; empties the queue.
@@ -888,10 +904,11 @@ __device__ __forceinline__ void b2_drain(float (&tv)[K], int (&ti)[K], const flo
 
 // Fold a finished 192 x 32 wave tile. Lane (h, c) holds column c and rows
 // 8 q + 4 h + rr of every 32 x 32 tile; lane c (h = 0) owns the column's
-// list. Marked scores are staged in the owner's scratch row and queued.
+// list. Marked scores are staged in the owner's scratch row and queued. Rows
+// below `lo` were scored by the tile before (see b2_row0) and are not offered.
 template <int K>
 __device__ __forceinline__ void b2_fold(const floatx16 (&acc)[6], float (&tv)[K], int (&ti)[K], int dbase,
-                                        int n_valid, float* scratch, int lane, float2* qbuf, int& qlen) {
+                                        int lo, float* scratch, int lane, float2* qbuf, int& qlen) {
   const int h = lane >> 5, c = lane & 31;
   // the column's K-th best (lane c's) in both lane halves
   const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(tv[K - 1]), __float_as_uint(tv[K - 1]), false,
@@ -936,14 +953,14 @@ __device__ __forceinline__ void b2_fold(const floatx16 (&acc)[6], float (&tv)[K]
         const int rho = __builtin_ctz(m);
         m &= m - 1;
         const int row = 32 * i + rho;
-        topk_offer_ascending<K>(tv, ti, row < n_valid ? scratch[BT_SCR_ROW * c + rho] : -INFINITY, dbase + row);
+        topk_offer_ascending<K>(tv, ti, row >= lo ? scratch[BT_SCR_ROW * c + rho] : -INFINITY, dbase + row);
       }
     } else {
       while (m != 0) {
         const int rho = __builtin_ctz(m);
         m &= m - 1;
         const int row = 32 * i + rho;
-        if (row < n_valid) {
+        if (row >= lo) {
           qbuf[32 * qlen + c] = make_float2(scratch[BT_SCR_ROW * c + rho], __int_as_float(dbase + row));
           ++qlen;
         }
@@ -994,7 +1011,7 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
   constexpr int NK = BT_RQ_ROW_BYTES / BT_KB;  // 4 stages per tile
   constexpr int LOADS = BT_RQ_LOADS;
   constexpr int STAGE = BT_RQ_STAGE;
-  constexpr int NS = DT == EBSD_DTYPE_FP8_E4M3 ? 1 : 2;
+  static_assert(DT == EBSD_DTYPE_FP8_E4M3, "the two-workgroup kernel takes FP8 rows with per-row scales");
   // 64 slots per XCD (two workgroups per CU); the mapping is slot_work's
   const SlotWork wk = slot_work(sc, blockIdx.x, (N + BT_D - 1) / BT_D);
   if (wk.idle) return;
@@ -1025,26 +1042,24 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
     floatx16 acc[6];
     int qlen = 0;
 
-    auto scale_src = [&](int t) -> const void* {
-      if constexpr (DT == EBSD_DTYPE_FP8_E4M3)
-        return d_scale + min(t * BT_D + 64 * wave + lane, N - 1);
-      else
-        return Dm + (size_t)min(t * BT_D, N - 1) * ld_bytes + 4 * lane;
-    };
     auto scale_m0 = [&](int t) { return lds0 + B2_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
-    // loads of stage g: k < 3 dictionary rows 16 (wave + 4 k) .., k = 3 the
-    // scales of the tile after g's; addresses clamped, also for g past the end
+    // Loads of stage g: k < 3 dictionary rows 16 (wave + 4 k) .. of its tile,
+    // k = 3 the scales of the tile after g's. Every tile is read from a
+    // uniform base (its first row, b2_row0) at the lane's fixed byte offset;
+    // rows of BT_RQ_ROW_BYTES, so the offsets fit 32 bits (see plan_sim).
+    auto issue_scales = [&](int t) {
+      bt_dma4_s(d_scale + b2_row0(t, N), (unsigned)tid * 4u, scale_m0(t));
+    };
     auto issue_part = [&](int g, int k) {
       const int tl = g / NK;
       const int kt = g - tl * NK;
       const int t = t_begin + tl;
       if (k < 3) {
-        const int L = wave + 4 * k;
-        const unsigned char* src =
-            Dm + (size_t)min(t * BT_D + 16 * L + lrow, N - 1) * ld_bytes + lchunk * 16 + kt * BT_KB;
-        bt_dma16(src, lds0 + (g % BT_NBUF) * STAGE + (wave_s + 4 * k) * 1024);
+        const unsigned off = (unsigned)((16 * (wave + 4 * k) + lrow) * BT_RQ_ROW_BYTES + lchunk * 16);
+        bt_dma16_s(Dm + (size_t)b2_row0(t, N) * BT_RQ_ROW_BYTES + kt * BT_KB, off,
+                   lds0 + (g % BT_NBUF) * STAGE + (wave_s + 4 * k) * 1024);
       } else {
-        bt_dma4(scale_src(t + 1), scale_m0(t + 1));
+        issue_scales(t + 1);
       }
     };
     auto dict_scales = [&](int t, int (&s)[2]) {
@@ -1080,7 +1095,7 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
       // every wave has its query operands before the ring overwrites them
       bt_wait_barrier<0>();
       // the first tile's scales and stages 0 .. 2
-      bt_dma4(scale_src(t_begin), scale_m0(t_begin));
+      issue_scales(t_begin);
 #pragma unroll
       for (int p = 0; p < BT_NBUF - 1; ++p)
 #pragma unroll
@@ -1110,7 +1125,8 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
       stage(tl, std::integral_constant<int, 2>{}, std::false_type{});
       stage(tl, std::integral_constant<int, 3>{}, std::false_type{});
       const int d0 = (t_begin + tl) * BT_D;
-      b2_fold<K>(acc, tv, ti, index_base + d0, N - d0, scratch, lane, qbuf, qlen);
+      const int r0 = b2_row0(t_begin + tl, N);
+      b2_fold<K>(acc, tv, ti, index_base + r0, d0 - r0, scratch, lane, qbuf, qlen);
       if ((tl % BT_DRAIN_TILES == BT_DRAIN_TILES - 1 || tl + 1 == n_t) && __builtin_amdgcn_ballot_w64(qlen > 0) != 0)
         b2_drain<K>(tv, ti, qbuf, qlen, r32);
       if (tl + 1 < n_t) dict_scales(t_begin + tl + 1, sa);
@@ -1265,7 +1281,9 @@ SimPlan plan_sim(int m, int n, int topk, int dtype, int ld_bytes) {
   p.k_list = topk <= 1 ? 1 : EBSD_MAX_TOPK;
   SimSched& s = p.sc;
   p.bt = dtype != EBSD_DTYPE_F32 && n >= BT_D && ld_bytes / BT_KB >= BT_MIN_STAGES;
-  p.b2 = p.bt && ld_bytes == BT_RQ_ROW_BYTES && dtype == EBSD_DTYPE_FP8_E4M3;
+  // (the two-workgroup kernel addresses its rows by 32-bit byte offsets)
+  p.b2 = p.bt && ld_bytes == BT_RQ_ROW_BYTES && dtype == EBSD_DTYPE_FP8_E4M3 &&
+         (long long)n * BT_RQ_ROW_BYTES <= 0x7fffffffLL;
   if (p.b2) {
     s.n_qtiles = (m + B2_Q - 1) / B2_Q;
     plan_groups(s, B2_SLOTS_PER_XCD, B2_SLOTS_PER_XCD, (n + BT_D - 1) / BT_D, p.n_parts);

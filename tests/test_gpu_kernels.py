@@ -282,6 +282,37 @@ def test_sim_topk_large_tile_kernel(dtype, m, n, d, k):
     _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
 
 
+@pytest.mark.parametrize("m,n,k", [(300, 193, 16), (1000, 383, 16), (257, 5000 + 7, 16), (4096, 20000 + 100, 4),
+                                   (129, 45000 + 191, 1)])
+def test_sim_topk_two_workgroup_kernel_partial_last_tile(m, n, k):
+    """FP8 rows of 256 bytes (the two-workgroup kernel): the last, partial
+    dictionary tile is loaded as the dictionary's last 192 rows, so its
+    leading rows are read a second time. Rows planted in that overlap with
+    the largest scores of every query must be listed exactly once, in their
+    place; lists equal the float64 brute force of the stored values."""
+    _say(f"sim_topk two-workgroup partial tile m={m} n={n} k={k}")
+    plan = kernels.sim_schedule(m, n, k, DTYPE_FP8_E4M3, 256, counts=False)
+    assert plan["bt"] == 2
+    g = torch.Generator().manual_seed(m + n + k)
+    d = 256
+    q = torch.randn(m, d, generator=g)
+    dd = torch.randn(n, d, generator=g)
+    lo, hi = max(n - 192, 0), (n // 192) * 192  # rows the last tile re-reads
+    if hi > lo:
+        plant = torch.arange(lo, hi, max((hi - lo) // 3, 1))[:3]
+        dd[plant] *= 10.0  # in the lists of a good share of the queries
+        dd[plant[0]] = 20.0 * q[0]  # first of query 0's list
+    Q = kernels.prep_patterns(q.to(DEV), DTYPE_FP8_E4M3, zero_mean=False)
+    D = kernels.prep_patterns(dd.to(DEV), DTYPE_FP8_E4M3, zero_mean=False)
+    val = torch.empty(m, k, device=DEV)
+    idx = torch.empty(m, k, dtype=torch.int32, device=DEV)
+    kernels.sim_topk(Q, D, k, val, idx, merge=False)
+    ix = idx.long().cpu()
+    for r in range(m):
+        assert len(set(ix[r].tolist())) == k, (r, ix[r].tolist())
+    _check_topk(val.cpu(), idx.cpu(), Q.to_float().cpu(), D.to_float().cpu(), k, tol=2e-5)
+
+
 @pytest.mark.parametrize("dtype,d,k", [(DTYPE_FP8_E4M3, 256, 16), (DTYPE_BF16, 128, 16), (DTYPE_FP8_E4M3, 384, 16),
                                        (DTYPE_FP8_E4M3, 256, 1), (DTYPE_F16, 256, 4)])
 def test_sim_topk_large_tile_exact_ties_over_many_tiles(dtype, d, k):
