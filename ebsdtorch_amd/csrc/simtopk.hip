@@ -318,6 +318,19 @@ __device__ __forceinline__ void bt_wait_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// As bt_wait_barrier, but up to L of this wave's LDS reads may stay in flight
+// across the barrier (reads of a buffer that no load of the next stage
+// overwrites).
+template <int N, int L>
+__device__ __forceinline__ void bt_wait_barrier_l() {
+  static_assert(N >= 0 && N < 64 && L >= 0 && L < 16, "vmcnt / lgkmcnt range");
+  constexpr int imm = (N & 15) | ((N >> 4) << 14) | (7 << 4) | (L << 8);
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(imm);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // MFMA operands of one stage for one wave: A = all 192 dictionary rows (six
 // 32-row blocks), B = its 64 queries (two blocks), 32 bytes of K per lane
 // and block.
@@ -396,6 +409,21 @@ __device__ __forceinline__ void bt_mfma_hooked(const intx8 (&a)[6], const intx8 
     bt_mfma_step<DT, S, IJ, IJ + 1, Z && S == 0>(a, b, sa, sb, acc);
     hook(std::integral_constant<int, K>{});
     bt_mfma_hooked<DT, Z, K + 1>(a, b, sa, sb, acc, hook);
+  }
+}
+
+// As bt_mfma_hooked, with the MFMAs of a K step in query-block-major order
+// (K = 12 S + 6 J + I): query block 0's operands are done with after MFMA
+// 12 (NS - 1) + 5, dictionary block i's after 12 (NS - 1) + 6 + i.
+template <int DT, bool Z, int K = 0, typename Hook>
+__device__ __forceinline__ void bt_mfma_hooked_jm(const intx8 (&a)[6], const intx8 (&b)[2], const int (&sa)[2], int sb,
+                                                  floatx16 (&acc)[6][2], Hook&& hook) {
+  constexpr int NS = DT == EBSD_DTYPE_FP8_E4M3 ? 1 : 2;
+  if constexpr (K < 12 * NS) {
+    constexpr int S = K / 12, J = (K % 12) / 6, I = K % 6;
+    bt_mfma_step<DT, S, 2 * I + J, 2 * I + J + 1, Z && S == 0>(a, b, sa, sb, acc);
+    hook(std::integral_constant<int, K>{});
+    bt_mfma_hooked_jm<DT, Z, K + 1>(a, b, sa, sb, acc, hook);
   }
 }
 
@@ -487,18 +515,26 @@ __device__ __forceinline__ void bt_drain(float (&tv)[K], int (&ti)[K], const flo
   qlen = 0;
 }
 
+// First dictionary row that tile t's loads read: the tile's own, except for
+// the last, partial tile, which reads the dictionary's last BT_D rows (the
+// leading ones were scored by the tile before and are skipped in the fold);
+// loads past the end read that last tile again. Every tile is thus read from
+// a uniform base at fixed per-lane offsets. Requires N >= BT_D.
+__device__ __forceinline__ int bt_row0(int t, int N) { return min(t * BT_D, N - BT_D); }
+
 // Fold a finished 192 x 64 wave tile into the lane's list. Lane (h, c)
 // (h = lane / 32, c = lane % 32) holds, of every 32 x 32 tile, column c and
 // rows 8 q + 4 h + rr (accumulator register 4 q + rr); its list belongs to
 // column c of column block h. dbase: index of the tile's first dictionary
-// row; n_valid: how many of its 192 rows exist; scratch: the wave's 64
+// row; lo: its first row not scored before (rows below lo were scored by
+// the tile before, see bt_row0); scratch: the wave's 64
 // scratch rows. Offers reach the list in ascending dictionary index. QUEUE:
 // marked scores go to the lane's offer queue (qbuf, qlen) instead of straight
 // into the list, and the bound of a column is its list's K-th best without the
 // queued offers (a lower bound, so nothing that belongs is screened out).
 template <int K, bool QUEUE>
 __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)[K], int (&ti)[K], int dbase,
-                                        int n_valid, float* scratch, int lane, float2* qbuf, int& qlen) {
+                                        int lo, float* scratch, int lane, float2* qbuf, int& qlen) {
   const int h = lane >> 5, c = lane & 31;
   // the K-th best scores of the two columns whose scores the lane holds:
   // column c of block 0 (the list of lane c) and of block 1 (lane 32 + c)
@@ -575,7 +611,7 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)
         const int rho = __builtin_ctz(m);
         m &= m - 1;
         const int row = 32 * i + rho;
-        if (row < n_valid) {
+        if (row >= lo) {
           qbuf[64 * qlen + lane] = make_float2(scratch[BT_SCR_ROW * lane + rho], __int_as_float(dbase + row));
           ++qlen;
         }
@@ -592,8 +628,8 @@ __device__ __forceinline__ void bt_fold(const floatx16 (&acc)[6][2], float (&tv)
       const int rho_next = __builtin_ctz(m | 0x80000000u);
       const float v_next = scratch[BT_SCR_ROW * lane + rho_next];
       const int row = 32 * i + rho;
-      // a row past the dictionary's end is offered as -inf (no change)
-      topk_offer_ascending<K>(tv, ti, row < n_valid ? v : -INFINITY, dbase + row);
+      // a row scored by the tile before is offered as -inf (no change)
+      topk_offer_ascending<K>(tv, ti, row >= lo ? v : -INFINITY, dbase + row);
       rho = rho_next;
       v = v_next;
     }
@@ -639,11 +675,13 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     // (a part with no tiles, G == 0, still writes its empty lists)
     const int q0 = qt * BT_Q;
     // this wave's four query loads of every stage: query rows
-    // 16 (wave + 4 j) + lrow, j < 4 (loads L = 12 + wave + 4 j)
-    const unsigned char* qsrc[4];
+    // 16 (wave + 4 j) + lrow, j < 4 (loads L = 12 + wave + 4 j), clamped, as
+    // byte offsets from the unit's first row qbase (stage kt: qbase + 64 kt)
+    const unsigned char* qbase = Q + (size_t)q0 * ld_bytes;
+    unsigned qoff[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      qsrc[j] = Q + (size_t)min(q0 + 16 * (wave + 4 * j) + lrow, M - 1) * ld_bytes + lchunk * 16;
+      qoff[j] = (unsigned)(min(16 * (wave + 4 * j) + lrow, M - 1 - q0) * ld_bytes + lchunk * 16);
     // FP8: E8M0 exponents of the lane's query rows, byte j for block j
     int sb = 0;
     if constexpr (DT == EBSD_DTYPE_FP8_E4M3) {
@@ -663,13 +701,13 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     // All addresses are clamped, also for g >= G; the buffer is g % BT_NBUF.
     const unsigned lds0 = (unsigned)(uintptr_t)(lds_ptr_t)smem;
     const unsigned wave_s = __builtin_amdgcn_readfirstlane(wave);
-    auto scale_src = [&](int t) -> const void* {
-      if constexpr (DT == EBSD_DTYPE_FP8_E4M3)
-        return d_scale + min(t * BT_D + 64 * wave + lane, N - 1);
-      else
-        return Dm + (size_t)min(t * BT_D, N - 1) * ld_bytes + 4 * lane;
-    };
     auto scale_m0 = [&](int t) { return lds0 + BT_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
+    auto issue_scales = [&](int t) {
+      if constexpr (DT == EBSD_DTYPE_FP8_E4M3)
+        bt_dma4_s(d_scale + bt_row0(t, N), (unsigned)tid * 4u, scale_m0(t));
+      else
+        bt_dma4_s(Dm + (size_t)bt_row0(t, N) * ld_bytes, (unsigned)lane * 4u, scale_m0(t));
+    };
     // RQ: the loads of stage g are its three dictionary parts and the scales
     // (k = 0 .. 3); the query rows are loaded once per work unit (qload)
     constexpr int LOADS = RQ ? BT_RQ_LOADS : BT_LOADS;
@@ -680,14 +718,12 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
       const int t = t_begin + tl;
       const unsigned mb = lds0 + (g % BT_NBUF) * STAGE;
       if (k < 3) {
-        const int L = wave + 4 * k;
-        const unsigned char* src =
-            Dm + (size_t)min(t * BT_D + 16 * L + lrow, N - 1) * ld_bytes + lchunk * 16 + kt * BT_KB;
-        bt_dma16(src, mb + (wave_s + 4 * k) * 1024);
+        const unsigned off = (unsigned)((16 * (wave + 4 * k) + lrow) * ld_bytes + lchunk * 16);
+        bt_dma16_s(Dm + (size_t)bt_row0(t, N) * ld_bytes + kt * BT_KB, off, mb + (wave_s + 4 * k) * 1024);
       } else if (!RQ && k < 7) {
-        bt_dma16(qsrc[k - 3] + kt * BT_KB, mb + (12 + wave_s + 4 * (k - 3)) * 1024);
+        bt_dma16_s(qbase + kt * BT_KB, qoff[k - 3], mb + (12 + wave_s + 4 * (k - 3)) * 1024);
       } else {
-        bt_dma4(scale_src(t + 1), scale_m0(t + 1));
+        issue_scales(t + 1);
       }
     };
     auto issue = [&](int g) {
@@ -701,7 +737,7 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
       for (int kt = 0; kt < 4; ++kt)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          bt_dma16(qsrc[j] + kt * BT_KB, lds0 + BT_RQ_QOFF + kt * BT_RQ_QSTAGE + (wave_s + 4 * j) * 1024);
+          bt_dma16_s(qbase + kt * BT_KB, qoff[j], lds0 + BT_RQ_QOFF + kt * BT_RQ_QSTAGE + (wave_s + 4 * j) * 1024);
     };
     // E8M0 exponents of the lane's dictionary rows of tile t (byte i % 4 of
     // s[i / 4]: block i)
@@ -721,7 +757,8 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     int qlen = 0;
     auto fold = [&](int tl) {
       const int d0 = (t_begin + tl) * BT_D;
-      bt_fold<K, RQ>(acc, tv, ti, index_base + d0, N - d0, scratch, lane, qbuf, qlen);
+      const int r0 = bt_row0(t_begin + tl, N);
+      bt_fold<K, RQ>(acc, tv, ti, index_base + r0, d0 - r0, scratch, lane, qbuf, qlen);
       if constexpr (RQ) {
         if ((tl % BT_DRAIN_TILES == BT_DRAIN_TILES - 1 || tl + 1 == n_t) && __builtin_amdgcn_ballot_w64(qlen > 0) != 0)
           bt_drain<K>(tv, ti, qbuf, qlen, lane);
@@ -734,29 +771,45 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
       __builtin_amdgcn_s_waitcnt(0xC07F);
     };
     if constexpr (!RQ) {
-      // Stage g is complete once the two batches issued after it are the
-      // only ones in flight (loads complete in issue order); the barrier then
-      // also means every wave is done reading the stages before g.
-      // prologue: the first tile's scales and stages 0 .. 2 in flight
+      // Operands one stage ahead in registers: during stage g the MFMAs use
+      // stage g's operands (read during stage g - 1) and the hooks read stage
+      // g + 1's into the registers as their last MFMA of stage g issues; the
+      // loads of stage g go to stage g + 3, into the buffer of stage g - 1,
+      // whose reads the MFMAs of stage g - 1 consumed. Stage g's barrier
+      // therefore means: stage g + 1 has landed (one batch stays in flight)
+      // and every wave is done with stage g - 1's buffer; the reads of stage
+      // g's own buffer may still be in flight.
+      // prologue: the first tile's scales and stages 0 .. 2 in flight, then
+      // stage 0's operands
+      BtFrag f;
       if (G > 0) {
-        bt_dma4(scale_src(t_begin), scale_m0(t_begin));
+        issue_scales(t_begin);
         for (int p = 0; p < BT_NBUF - 1; ++p) issue(p);
         bt_wait_barrier<2 * LOADS>();
         dict_scales(t_begin, sa);
-        __builtin_amdgcn_s_waitcnt(0xC07F);
+        bt_read<DT>(smem, smem + BT_D * BT_KB, wq, lane, f);
       }
+      constexpr int NS = DT == EBSD_DTYPE_FP8_E4M3 ? 1 : 2;
+      constexpr int NMF = 12 * NS;
+      constexpr int LAST = 12 * (NS - 1);  // first MFMA of the stage's last K step
       for (int tl = 0; tl < n_t; ++tl) {
-        // stage g of the tile; first: the tile's first stage
         auto stage = [&](int g, auto first) {
-          bt_wait_barrier<2 * LOADS>();
-          BtFrag f;
-          const unsigned char* abuf = smem + (g % BT_NBUF) * STAGE;
-          bt_read<DT>(abuf, abuf + BT_D * BT_KB, wq, lane, f);
-          __builtin_amdgcn_sched_barrier(0);
-          // the MFMAs start as their operands come in; the loads of stage
-          // g + 3 (into the buffer of stage g - 1) go out between them
-          bt_mfma_hooked<DT, decltype(first)::value>(f.a, f.b, sa, sb, acc, [&](auto k) {
-            if constexpr (decltype(k)::value < LOADS) issue_part(g + BT_NBUF - 1, decltype(k)::value);
+          bt_wait_barrier_l<LOADS, 15>();
+          const unsigned char* nbuf = smem + ((g + 1) % BT_NBUF) * STAGE;
+          bt_mfma_hooked_jm<DT, decltype(first)::value>(f.a, f.b, sa, sb, acc, [&](auto k) {
+            constexpr int kk = decltype(k)::value;
+            // next-stage operands into the registers this MFMA was the last to use
+            if constexpr (kk == LAST + 5) f.b[0] = bt_rd<DT>(nbuf + BT_D * BT_KB, BT_WQ * wq + r32, lane);
+            if constexpr (kk >= LAST + 6) {
+              f.a[kk - LAST - 6] = bt_rd<DT>(nbuf, 32 * (kk - LAST - 6) + r32, lane);
+              if constexpr (kk == LAST + 11) f.b[1] = bt_rd<DT>(nbuf + BT_D * BT_KB, BT_WQ * wq + 32 + r32, lane);
+            }
+            // part j of stage g + 3's loads goes out after MFMA j * NMF / LOADS
+            constexpr int j = (kk * LOADS + NMF - 1) / NMF;
+            if constexpr (j < LOADS && j * NMF / LOADS == kk) {
+              issue_part(g + BT_NBUF - 1, j);
+              __builtin_amdgcn_sched_barrier(0);
+            }
           });
         };
         stage(tl * nk, std::true_type{});
@@ -775,7 +828,7 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
       intx8 bres[4][2];
       intx8 a[6];
       if (G > 0) {
-        bt_dma4(scale_src(t_begin), scale_m0(t_begin));
+        issue_scales(t_begin);
         qload();
         for (int p = 0; p < BT_NBUF; ++p) issue(p);
         bt_wait_barrier<(BT_NBUF - 1) * LOADS>();
@@ -883,12 +936,6 @@ __device__ __forceinline__ float b2_max16(const floatx16& a) {
   return __builtin_elementwise_maximum(b2_max3(r0, r1, r2), b2_max3(r3, r4, a[15]));
 }
 
-// First dictionary row that tile t's loads read: the tile's own, except for
-// the last, partial tile, which reads the dictionary's last BT_D rows (the
-// leading ones were scored by the tile before and are skipped in the fold);
-// loads past the end read that last tile again. Requires N >= BT_D.
-__device__ __forceinline__ int b2_row0(int t, int N) { return min(t * BT_D, N - BT_D); }
-
 // Offer the lane's queued entries (slot s at qbuf[32 s + c]) to its list in
 // queue order, the next one read ahead; empties the queue.
 template <int K>
@@ -905,7 +952,7 @@ __device__ __forceinline__ void b2_drain(float (&tv)[K], int (&ti)[K], const flo
 // Fold a finished 192 x 32 wave tile. Lane (h, c) holds column c and rows
 // 8 q + 4 h + rr of every 32 x 32 tile; lane c (h = 0) owns the column's
 // list. Marked scores are staged in the owner's scratch row and queued. Rows
-// below `lo` were scored by the tile before (see b2_row0) and are not offered.
+// below `lo` were scored by the tile before (see bt_row0) and are not offered.
 template <int K>
 __device__ __forceinline__ void b2_fold(const floatx16 (&acc)[6], float (&tv)[K], int (&ti)[K], int dbase,
                                         int lo, float* scratch, int lane, float2* qbuf, int& qlen) {
@@ -1045,10 +1092,9 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
     auto scale_m0 = [&](int t) { return lds0 + B2_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
     // Loads of stage g: k < 3 dictionary rows 16 (wave + 4 k) .. of its tile,
     // k = 3 the scales of the tile after g's. Every tile is read from a
-    // uniform base (its first row, b2_row0) at the lane's fixed byte offset;
-    // rows of BT_RQ_ROW_BYTES, so the offsets fit 32 bits (see plan_sim).
+    // uniform base (its first row, bt_row0) at the lane's fixed byte offset.
     auto issue_scales = [&](int t) {
-      bt_dma4_s(d_scale + b2_row0(t, N), (unsigned)tid * 4u, scale_m0(t));
+      bt_dma4_s(d_scale + bt_row0(t, N), (unsigned)tid * 4u, scale_m0(t));
     };
     auto issue_part = [&](int g, int k) {
       const int tl = g / NK;
@@ -1056,7 +1102,7 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
       const int t = t_begin + tl;
       if (k < 3) {
         const unsigned off = (unsigned)((16 * (wave + 4 * k) + lrow) * BT_RQ_ROW_BYTES + lchunk * 16);
-        bt_dma16_s(Dm + (size_t)b2_row0(t, N) * BT_RQ_ROW_BYTES + kt * BT_KB, off,
+        bt_dma16_s(Dm + (size_t)bt_row0(t, N) * BT_RQ_ROW_BYTES + kt * BT_KB, off,
                    lds0 + (g % BT_NBUF) * STAGE + (wave_s + 4 * k) * 1024);
       } else {
         issue_scales(t + 1);
@@ -1107,7 +1153,9 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
     }
     auto stage = [&](int tl, auto kt, auto first) {
       const int g = NK * tl + decltype(kt)::value;
-      bt_wait_barrier<LOADS>();
+      // the next stage's first two blocks (four reads of buffer g) may still
+      // be in flight: no load of stage g writes that buffer
+      bt_wait_barrier_l<LOADS, 2 * B2_ASLOTS>();
       const unsigned char* cbuf = smem + (g % BT_NBUF) * STAGE;
       const unsigned char* nbuf = smem + ((g + 1) % BT_NBUF) * STAGE;
       b2_mfma_step<DT, 0, decltype(first)::value>(ar, bres[decltype(kt)::value], sa, sb, acc, [&](auto i) {
@@ -1125,7 +1173,7 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
       stage(tl, std::integral_constant<int, 2>{}, std::false_type{});
       stage(tl, std::integral_constant<int, 3>{}, std::false_type{});
       const int d0 = (t_begin + tl) * BT_D;
-      const int r0 = b2_row0(t_begin + tl, N);
+      const int r0 = bt_row0(t_begin + tl, N);
       b2_fold<K>(acc, tv, ti, index_base + r0, d0 - r0, scratch, lane, qbuf, qlen);
       if ((tl % BT_DRAIN_TILES == BT_DRAIN_TILES - 1 || tl + 1 == n_t) && __builtin_amdgcn_ballot_w64(qlen > 0) != 0)
         b2_drain<K>(tv, ti, qbuf, qlen, r32);
@@ -1281,9 +1329,7 @@ SimPlan plan_sim(int m, int n, int topk, int dtype, int ld_bytes) {
   p.k_list = topk <= 1 ? 1 : EBSD_MAX_TOPK;
   SimSched& s = p.sc;
   p.bt = dtype != EBSD_DTYPE_F32 && n >= BT_D && ld_bytes / BT_KB >= BT_MIN_STAGES;
-  // (the two-workgroup kernel addresses its rows by 32-bit byte offsets)
-  p.b2 = p.bt && ld_bytes == BT_RQ_ROW_BYTES && dtype == EBSD_DTYPE_FP8_E4M3 &&
-         (long long)n * BT_RQ_ROW_BYTES <= 0x7fffffffLL;
+  p.b2 = p.bt && ld_bytes == BT_RQ_ROW_BYTES && dtype == EBSD_DTYPE_FP8_E4M3;
   if (p.b2) {
     s.n_qtiles = (m + B2_Q - 1) / B2_Q;
     plan_groups(s, B2_SLOTS_PER_XCD, B2_SLOTS_PER_XCD, (n + BT_D - 1) / BT_D, p.n_parts);
