@@ -211,11 +211,18 @@ __global__ __launch_bounds__(256, 2) void sim_topk_kernel(
 //   stages  : 64 bytes of K for all 448 rows (28 KiB) land in LDS straight
 //             from global memory (global_load_lds_dwordx4, 7 per wave per
 //             stage, plus one 4-byte load of the next tile's dictionary
-//             scales); stage g + 3 is issued as soon as every wave has read
-//             stage g - 1, so three stages are in flight while one is read.
-//             Issuing is branch-free: past the last stage the loads fetch
-//             clamped rows into the free buffer, so every stage waits for the
-//             same count and the MFMAs can be scheduled across the issue.
+//             scales), spread over the stage's MFMAs. Each wave reads stage
+//             g + 1's operands into registers while stage g's MFMAs run
+//             (each register as soon as its last MFMA of stage g issues), so
+//             a stage starts with its operands in hand; stage g + 3 goes into
+//             the buffer of stage g - 1, which the MFMAs of stage g - 1 have
+//             consumed. Every tile is read from a uniform base (its first
+//             row) at fixed per-lane offsets; the last, partial tile is read
+//             as the dictionary's last 192 rows, and the rows it shares with
+//             the tile before are skipped in the fold. Issuing is
+//             branch-free: past the last stage the loads fetch the last tile
+//             again into the free buffer, so every stage waits for the same
+//             count and the MFMAs can be scheduled across the issue.
 //             Rows of 256 bytes (4 stages) keep the work unit's 256 query
 //             rows resident in LDS instead (loaded once, 64 KiB), and the
 //             ring carries the 192 dictionary rows only (12 KiB, 3 loads per
