@@ -30,6 +30,7 @@ namespace {
 using namespace ebsd_lambert;
 
 constexpr float kFp8Max = 448.0f;  // largest finite OCP e4m3fn value
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 __device__ __forceinline__ T to_out(float v);
@@ -47,12 +48,39 @@ __device__ __forceinline__ uint8_t to_out<uint8_t>(float v) {
 template <typename T> struct is_fp8 { static constexpr bool value = false; };
 template <> struct is_fp8<uint8_t> { static constexpr bool value = true; };
 
+// Register slot layouts of a row: V = 1, slot i of thread t is column
+// t + i BLOCK (one value per thread and load); V = 4, slot i is column
+// 4 (t + (i / 4) BLOCK) + i % 4 (four consecutive columns per thread and
+// 16-byte load or store). Both cover columns [0, ITEMS BLOCK).
+template <int V, int BLOCK>
+__device__ __forceinline__ int slot_col(int i) {
+  if constexpr (V == 1)
+    return (int)threadIdx.x + i * BLOCK;
+  else
+    return 4 * ((int)threadIdx.x + (i / 4) * BLOCK) + (i & 3);
+}
+
+// Four consecutive outputs of type T as one store.
+template <typename T>
+__device__ __forceinline__ void store4(T* p, float a, float b, float c, float d) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
+  } else if constexpr (sizeof(T) == 2) {
+    const T v[4] = {to_out<T>(a), to_out<T>(b), to_out<T>(c), to_out<T>(d)};
+    *reinterpret_cast<uint2*>(p) = *reinterpret_cast<const uint2*>(v);
+  } else {
+    const T v[4] = {to_out<T>(a), to_out<T>(b), to_out<T>(c), to_out<T>(d)};
+    *reinterpret_cast<unsigned*>(p) = *reinterpret_cast<const unsigned*>(v);
+  }
+}
+
 // Normalise the ITEMS register values of one pattern (n_valid of them are
 // real pixels) and store row `row` of `out`, zero-padding to ld.
-template <int ITEMS, typename OutT, int BLOCK = 256>
+template <int ITEMS, typename OutT, int BLOCK = 256, int V = 1>
 __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, int flags,
                                            OutT* __restrict__ out, int ld, float* __restrict__ out_scale,
                                            size_t row, float* red) {
+  static_assert(V == 1 || (V == 4 && ITEMS % 4 == 0), "slot layout");
   const int tid = threadIdx.x;
   if (flags & EBSD_FLAG_ZERO_MEAN) {
     float s = 0.0f;
@@ -61,7 +89,7 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
     const float mean = block_sum<BLOCK>(s, red) / (float)n_valid;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const int d = tid + i * BLOCK;
+      const int d = slot_col<V, BLOCK>(i);
       vals[i] = (d < n_valid) ? (vals[i] - mean) : 0.0f;
     }
   }
@@ -95,10 +123,23 @@ __device__ __forceinline__ void finish_row(float (&vals)[ITEMS], int n_valid, in
     if (tid == 0 && out_scale) out_scale[row] = 1.0f;
   }
   OutT* orow = out + row * (size_t)ld;
+  if constexpr (V == 1) {
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const int d = tid + i * BLOCK;
-    if (d < ld) orow[d] = to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f);
+    for (int i = 0; i < ITEMS; ++i) {
+      const int d = tid + i * BLOCK;
+      if (d < ld) orow[d] = to_out<OutT>(d < n_valid ? vals[i] * mul : 0.0f);
+    }
+  } else {
+    // ld and n_valid are multiples of 4 here (see launch_prep)
+#pragma unroll
+    for (int i = 0; i < ITEMS; i += 4) {
+      const int d = slot_col<V, BLOCK>(i);
+      if (d < ld) {
+        const bool in = d < n_valid;
+        store4<OutT>(orow + d, in ? vals[i] * mul : 0.0f, in ? vals[i + 1] * mul : 0.0f,
+                     in ? vals[i + 2] * mul : 0.0f, in ? vals[i + 3] * mul : 0.0f);
+      }
+    }
   }
   // Columns beyond ITEMS*BLOCK (only when ld exceeds it) are zero padding.
   for (int d = ITEMS * BLOCK + tid; d < ld; d += BLOCK) orow[d] = to_out<OutT>(0.0f);
@@ -243,48 +284,80 @@ __global__ __launch_bounds__(BLOCK) void project_kernel(
 // One workgroup of BLOCK threads per row, ITEMS values per thread (rows of
 // up to 4,096 pixels: 256 threads; longer rows: 1,024 threads, so that a
 // thread holds at most 16 values and occupancy stays up).
-template <int ITEMS, int BLOCK, typename OutT>
+// V = 4: rows read and written 16 bytes per thread and access (unmasked rows
+// whose width, leading dimensions and base addresses are multiples of 4
+// values / 16 bytes; see launch_prep), V = 1 otherwise.
+template <int ITEMS, int BLOCK, int V>
+__device__ __forceinline__ void prep_load(float (&vals)[ITEMS], const float* __restrict__ row,
+                                          const int* __restrict__ pix_idx, int n_pix) {
+  if constexpr (V == 1) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int d = threadIdx.x + i * BLOCK;
+      float v = 0.0f;
+      if (d < n_pix) v = __builtin_nontemporal_load(&row[pix_idx ? pix_idx[d] : d]);  // read once
+      vals[i] = v;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; i += 4) {
+      const int d = slot_col<V, BLOCK>(i);
+      f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (d < n_pix) v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(row + d));  // read once
+      vals[i] = v[0];
+      vals[i + 1] = v[1];
+      vals[i + 2] = v[2];
+      vals[i + 3] = v[3];
+    }
+  }
+}
+
+template <int ITEMS, int BLOCK, typename OutT, int V>
 __global__ __launch_bounds__(BLOCK) void prep_kernel(
     const float* __restrict__ pats, int ld_in, const int* __restrict__ pix_idx, int n_pix,
     OutT* __restrict__ out, int ld, float* __restrict__ out_scale, int flags) {
   __shared__ float red[BLOCK / 64];
   const size_t n = blockIdx.x;
-  const float* row = pats + n * (size_t)ld_in;
   float vals[ITEMS];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const int d = threadIdx.x + i * BLOCK;
-    float v = 0.0f;
-    if (d < n_pix) v = __builtin_nontemporal_load(&row[pix_idx ? pix_idx[d] : d]);  // read once
-    vals[i] = v;
-  }
-  finish_row<ITEMS, OutT, BLOCK>(vals, n_pix, flags, out, ld, out_scale, n, red);
+  prep_load<ITEMS, BLOCK, V>(vals, pats + n * (size_t)ld_in, pix_idx, n_pix);
+  finish_row<ITEMS, OutT, BLOCK, V>(vals, n_pix, flags, out, ld, out_scale, n, red);
 }
 
 // prep_kernel with a second output: the finished FP32 row and the same
 // values converted to Out2 (BF16 / FP16), from one read of the patterns.
-template <int ITEMS, int BLOCK, typename Out2>
+template <int ITEMS, int BLOCK, typename Out2, int V>
 __global__ __launch_bounds__(BLOCK) void prep_pair_kernel(
     const float* __restrict__ pats, int ld_in, const int* __restrict__ pix_idx, int n_pix,
     float* __restrict__ out, int ld, Out2* __restrict__ out2, int ld2, int flags) {
   __shared__ float red[BLOCK / 64];
   const size_t n = blockIdx.x;
-  const float* row = pats + n * (size_t)ld_in;
   float vals[ITEMS];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const int d = threadIdx.x + i * BLOCK;
-    float v = 0.0f;
-    if (d < n_pix) v = __builtin_nontemporal_load(&row[pix_idx ? pix_idx[d] : d]);  // read once
-    vals[i] = v;
-  }
+  prep_load<ITEMS, BLOCK, V>(vals, pats + n * (size_t)ld_in, pix_idx, n_pix);
   // finish_row leaves the finished values in vals (FP32 output: no scale)
-  finish_row<ITEMS, float, BLOCK>(vals, n_pix, flags, out, ld, nullptr, n, red);
+  finish_row<ITEMS, float, BLOCK, V>(vals, n_pix, flags, out, ld, nullptr, n, red);
   Out2* orow = out2 + n * (size_t)ld2;
+  if constexpr (V == 1) {
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const int d = threadIdx.x + i * BLOCK;
-    if (d < ld2) orow[d] = to_out<Out2>(d < n_pix ? vals[i] : 0.0f);
+    for (int i = 0; i < ITEMS; ++i) {
+      const int d = threadIdx.x + i * BLOCK;
+      if (d < ld2) orow[d] = to_out<Out2>(d < n_pix ? vals[i] : 0.0f);
+    }
+  } else if ((ld2 & 3) == 0 && (reinterpret_cast<uintptr_t>(orow) & (4 * sizeof(Out2) - 1)) == 0) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; i += 4) {
+      const int d = slot_col<V, BLOCK>(i);
+      if (d < ld2) {
+        const bool in = d < n_pix;
+        store4<Out2>(orow + d, in ? vals[i] : 0.0f, in ? vals[i + 1] : 0.0f, in ? vals[i + 2] : 0.0f,
+                     in ? vals[i + 3] : 0.0f);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int d = slot_col<V, BLOCK>(i);
+      if (d < ld2) orow[d] = to_out<Out2>(d < n_pix ? vals[i] : 0.0f);
+    }
   }
   for (int d = ITEMS * BLOCK + threadIdx.x; d < ld2; d += BLOCK) orow[d] = to_out<Out2>(0.0f);
 }
@@ -545,16 +618,33 @@ int launch_project(const float* mp, int H, int W, const float* rays, int n_pix, 
   return EBSD_ERR_SHAPE;
 }
 
+// Rows take the 16-byte layout (V = 4) when they are read whole (no pixel
+// mask) and their width, row stride and base address are multiples of 4
+// values / 16 bytes. The layout only changes which thread holds which
+// values (and so the order of the row sums): prep_kernel and
+// prep_pair_kernel decide it alike, so their FP32 rows stay identical.
+inline bool prep_vec4(const float* pats, int ld_in, const int* pix_idx, int n_pix) {
+  return pix_idx == nullptr && n_pix % 4 == 0 && ld_in % 4 == 0 && (reinterpret_cast<uintptr_t>(pats) & 15) == 0;
+}
+
 template <typename Out2>
 int launch_prep_pair(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix, float* out, int ld,
                      void* out2, int ld2, int flags, hipStream_t st) {
   Out2* o2 = static_cast<Out2*>(out2);
-#define EBSD_PREP_PAIR_CASE(I, B)                                                                         \
-  if (n_pix <= (I) * (B)) {                                                                               \
-    hipLaunchKernelGGL((prep_pair_kernel<I, B, Out2>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx, n_pix, \
-                       out, ld, o2, ld2, flags);                                                          \
-    EBSD_LAUNCH_CHECK();                                                                                  \
-    return EBSD_OK;                                                                                       \
+  // the FP32 rows' layout is decided as launch_prep decides it for FP32
+  // output; the second output is stored 4 at a time where aligned
+  const bool v4 = prep_vec4(pats, ld_in, pix_idx, n_pix) && ld % 4 == 0 &&
+                  (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+#define EBSD_PREP_PAIR_CASE(I, B)                                                                            \
+  if (n_pix <= (I) * (B)) {                                                                                  \
+    if (v4)                                                                                                  \
+      hipLaunchKernelGGL((prep_pair_kernel<I, B, Out2, 4>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx,    \
+                         n_pix, out, ld, o2, ld2, flags);                                                    \
+    else                                                                                                     \
+      hipLaunchKernelGGL((prep_pair_kernel<I, B, Out2, 1>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx,    \
+                         n_pix, out, ld, o2, ld2, flags);                                                    \
+    EBSD_LAUNCH_CHECK();                                                                                     \
+    return EBSD_OK;                                                                                          \
   }
   EBSD_PREP_PAIR_CASE(4, 256)
   EBSD_PREP_PAIR_CASE(8, 256)
@@ -569,12 +659,18 @@ template <typename OutT>
 int launch_prep(const float* pats, int n, int ld_in, const int* pix_idx, int n_pix, void* out,
                 int ld, float* out_scale, int flags, hipStream_t st) {
   OutT* o = static_cast<OutT*>(out);
-#define EBSD_PREP_CASE(I, B)                                                                          \
-  if (n_pix <= (I) * (B)) {                                                                           \
-    hipLaunchKernelGGL((prep_kernel<I, B, OutT>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx, n_pix, \
-                       o, ld, out_scale, flags);                                                      \
-    EBSD_LAUNCH_CHECK();                                                                              \
-    return EBSD_OK;                                                                                   \
+  const bool v4 = prep_vec4(pats, ld_in, pix_idx, n_pix) && ld % 4 == 0 &&
+                  (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+#define EBSD_PREP_CASE(I, B)                                                                                 \
+  if (n_pix <= (I) * (B)) {                                                                                  \
+    if (v4)                                                                                                  \
+      hipLaunchKernelGGL((prep_kernel<I, B, OutT, 4>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx, n_pix,   \
+                         o, ld, out_scale, flags);                                                           \
+    else                                                                                                     \
+      hipLaunchKernelGGL((prep_kernel<I, B, OutT, 1>), dim3(n), dim3(B), 0, st, pats, ld_in, pix_idx, n_pix,   \
+                         o, ld, out_scale, flags);                                                           \
+    EBSD_LAUNCH_CHECK();                                                                                     \
+    return EBSD_OK;                                                                                          \
   }
   EBSD_PREP_CASE(4, 256)
   EBSD_PREP_CASE(8, 256)

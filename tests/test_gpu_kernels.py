@@ -117,10 +117,12 @@ def test_projection_low_precision_outputs(mp101, dtype):
         assert torch.equal(lp.data[:, :3600], f32.to(tdt))
 
 
-@pytest.mark.parametrize("p", [3600, 14400, 5000])
+@pytest.mark.parametrize("p", [3600, 14400, 5000, 3597])
 def test_prep_patterns_mean_mask_and_norm(p):
     """Rows of up to 4,096 pixels take 256-thread workgroups, longer rows
-    1,024 threads (8 or 16 values per thread)."""
+    1,024 threads (8 or 16 values per thread); widths that are multiples of
+    4 are read and written 16 bytes per access, others one value at a
+    time."""
     _say(f"prep patterns p={p}")
     g = torch.Generator().manual_seed(4 + p)
     x = torch.rand(37, p, generator=g) * 3 + 1
@@ -143,7 +145,7 @@ def test_prep_patterns_mean_mask_and_norm(p):
 
 @pytest.mark.parametrize("dtype2", [DTYPE_BF16, DTYPE_F16])
 @pytest.mark.parametrize("p,masked,unit_norm", [(3600, False, False), (3600, True, True), (100, False, True),
-                                                (14400, True, False)])
+                                                (14400, True, False), (3597, False, False)])
 def test_prep_patterns_pair_equals_two_preps(dtype2, p, masked, unit_norm):
     """ebsd_prep_patterns_pair writes bit for bit what two ebsd_prep_patterns
     calls (FP32 and dtype2) write, padding included."""
