@@ -703,15 +703,18 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
     // Loads of stage g, in 8 parts: k < 3 dictionary rows 16 (wave + 4 k) ..,
     // k = 3 .. 6 query rows 16 (wave + 4 (k - 3)) .., k = 7 the dictionary
     // scales of the tile after g's (tile t's into LDS slot t % 2: 256
-    // entries, 64 per wave, rows clamped, entries 192 .. 255 unused; BF16 /
-    // FP16 load bytes of the dictionary instead, so every batch has 8 loads).
-    // All addresses are clamped, also for g >= G; the buffer is g % BT_NBUF.
+    // entries, 64 per wave, entry e the scale of the tile's row min(e, 191):
+    // entries 192 .. 255 are unused and stay inside the tile; BF16 / FP16
+    // load bytes of the tile's first row instead, so every batch has 8
+    // loads). Every address lies in the dictionary or the query block (tiles
+    // from bt_row0, query rows clamped), also for g >= G; the buffer is
+    // g % BT_NBUF.
     const unsigned lds0 = (unsigned)(uintptr_t)(lds_ptr_t)smem;
     const unsigned wave_s = __builtin_amdgcn_readfirstlane(wave);
     auto scale_m0 = [&](int t) { return lds0 + BT_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
     auto issue_scales = [&](int t) {
       if constexpr (DT == EBSD_DTYPE_FP8_E4M3)
-        bt_dma4_s(d_scale + bt_row0(t, N), (unsigned)tid * 4u, scale_m0(t));
+        bt_dma4_s(d_scale + bt_row0(t, N), (unsigned)min(tid, BT_D - 1) * 4u, scale_m0(t));
       else
         bt_dma4_s(Dm + (size_t)bt_row0(t, N) * ld_bytes, (unsigned)lane * 4u, scale_m0(t));
     };
@@ -867,7 +870,7 @@ __global__ __launch_bounds__(BT_THREADS, 1) void sim_topk_bt_kernel(
         fold(tl);
       }
     }
-    // The clamped loads past the last stage land, and every wave is done with
+    // The loads past the last stage land, and every wave is done with
     // the stage ring, the scales and its scratch, before the next unit's
     // loads reuse the LDS (this wait includes lgkmcnt(0) and a barrier).
     bt_wait_barrier<0>();
@@ -1098,10 +1101,12 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
 
     auto scale_m0 = [&](int t) { return lds0 + B2_SCALE_OFF + ((t & 1) * 256 + 64 * wave_s) * 4; };
     // Loads of stage g: k < 3 dictionary rows 16 (wave + 4 k) .. of its tile,
-    // k = 3 the scales of the tile after g's. Every tile is read from a
-    // uniform base (its first row, bt_row0) at the lane's fixed byte offset.
+    // k = 3 the scales of the tile after g's (entry e of the 256: the scale
+    // of the tile's row min(e, 191), so every read stays inside the tile).
+    // Every tile is read from a uniform base (its first row, bt_row0) at the
+    // lane's fixed byte offset.
     auto issue_scales = [&](int t) {
-      bt_dma4_s(d_scale + bt_row0(t, N), (unsigned)tid * 4u, scale_m0(t));
+      bt_dma4_s(d_scale + bt_row0(t, N), (unsigned)min(tid, BT_D - 1) * 4u, scale_m0(t));
     };
     auto issue_part = [&](int g, int k) {
       const int tl = g / NK;
@@ -1187,7 +1192,7 @@ __global__ __launch_bounds__(BT_THREADS, 2) void sim_topk_b2_kernel(
       if (tl + 1 < n_t) dict_scales(t_begin + tl + 1, sa);
       __builtin_amdgcn_s_waitcnt(0xC07F);
     }
-    // the clamped loads past the last stage land and every wave is done with
+    // the loads past the last stage land and every wave is done with
     // the LDS before the next unit's query rows overwrite the ring
     bt_wait_barrier<0>();
     const int q = q0 + B2_WQ * wq + r32;
