@@ -720,7 +720,14 @@ def learned_form(a, mp, dico, dev):
     idx = LearnedIndexer(net, mp, rays, 11, fallback=fb, threshold=0.0, binning=binning,
                          local_rays=geom.get_rays((binning, binning)).to(dev), rescore_k=rescore_k)
     cal = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=6)
-    idx.threshold = calibrate_threshold(idx, cal, n=8192, max_error_deg=1.0, precision=0.999)
+    # the threshold keeps 99.99% of the accepted calibration patterns within
+    # 1 degree, on 32,768 of them. The rare wrong proposals (a wrong basin,
+    # tens of degrees off) score below ~0.6 while almost every right one
+    # scores above 0.7, so the strict target costs ~0.1% more fallback; a
+    # 99.9% target on 8,192 patterns allows ~8 misses and put the threshold
+    # anywhere in 0.08-0.38 from one seed to the next, letting the held-out
+    # batch fall below 99.9% within 1 degree
+    idx.threshold = calibrate_threshold(idx, cal, n=32768, max_error_deg=1.0, precision=0.9999)
     test = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=7)
     n = a.learned_batch
     x, q_true = test.batch(n)
@@ -1084,7 +1091,7 @@ def c5_workload(a, dev):
                           local_rays=geom.get_rays((c5_binning, c5_binning)).to(dev), rescore_k=8)
     lidx.threshold = calibrate_threshold(
         lidx, PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=21),
-        n=8192)
+        n=8192, precision=0.999)
     scan = PatternSimulator(mp, geom, mean_counts=(100.0, 400.0), sigma=0.05, pc_jitter=0.0, seed=22)
 
     t_di = t_le = 0.0
