@@ -251,7 +251,8 @@ int ebsd_sim_topk(const void* q, const float* q_scale, int m, const void* dict,
  * allocate (code under HIP graph capture, serving loops, harnesses that
  * forbid hipMalloc). Sizes whose partial lists exceed the buffer (FP32 with
  * top-k > 1: m > 65,536 queries; BF16 / FP16 / FP8 on the 192 x 256 kernel:
- * m > 262,144) return EBSD_ERR_SCRATCH before any device work. Calls that
+ * m > 262,144; FP8 rows of 256 bytes on the 192 x 128 two-workgroup kernel:
+ * m > 524,288) return EBSD_ERR_SCRATCH before any device work. Calls that
  * overlap on one GPU share the buffer and race: serialise them.
  * No counterpart in the reference.
  */
