@@ -93,8 +93,8 @@ def parse():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(the dictionary-indexing path shards by query batch with no exchange, see the design notes)",
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(the dictionary-indexing path would shard by query batch with no exchange between GPUs)",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -1,7 +1,7 @@
 """Write the reference golden vectors used to pin the CPU oracle.
 
-Run in the build container only (it needs the ebsdtorch v0.0.64 sources at
-/root/reference, which never travel to the GPU box):
+Run only where the ebsdtorch v0.0.64 sources sit at /root/reference (the
+machines that run the GPU tests do not have them):
 
     python tests/golden/make_golden.py
 
